@@ -1,0 +1,139 @@
+/*
+ * mdpd355.h -- C ABI of the MI355X engine for USER-MESO's per-timestep
+ * mDPD / DPD particle path (pair_style mdpd/meso and dpd/meso, fix nve/meso,
+ * fix aboundary/rho/meso, fix addforce/meso, run_style mvv/meso).
+ *
+ * The boundary keeps the shape of the reference's own embedding interface,
+ * the LAMMPS library API (src/library.h:27-42), restricted to the styles of
+ * this path: a handle is opened, driven with input-script commands, filled
+ * with atoms, run, and read back.  All functions return an MDPD_* status;
+ * the text of the last error of a handle is available from mdpd_last_error.
+ * No torch or HIP types cross the boundary: plain pointers and sizes only.
+ * The handle owns all device memory; host buffers passed in are copied.
+ * One host thread drives a handle at a time; distinct handles are
+ * independent.
+ */
+#ifndef MDPD355_H
+#define MDPD355_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MDPD_OK = 0,
+    MDPD_E_ARG = 1,         /* bad argument or script syntax (error->all in the reference) */
+    MDPD_E_STATE = 2,       /* call out of order (e.g. run before pair_coeff) */
+    MDPD_E_HIP = 3,         /* HIP runtime failure */
+    MDPD_E_OVERFLOW = 4,    /* a device table could not hold the system */
+    MDPD_E_NOMEM = 5,       /* allocation failed */
+    MDPD_E_UNSUPPORTED = 6  /* command or style outside this path */
+};
+
+typedef struct mdpd_system mdpd_system;
+
+/* ---- lifecycle -------------------------------------------------------- */
+
+/* Open a handle on HIP device `device`.  Replaces lammps_open
+ * (src/library.h:27-28) + MesoDevice::init (src/USER-MESO/engine_meso.cu:38-95). */
+int mdpd_open(int device, mdpd_system **out);
+/* Release the handle and its device memory.  lammps_close (src/library.h:29) */
+void mdpd_close(mdpd_system *sys);
+/* Text of the last error on this handle ("" if none).  The reference raises
+ * error->all / SIGABRT instead (src/USER-MESO/util_meso.h:100-101). */
+const char *mdpd_last_error(const mdpd_system *sys);
+const char *mdpd_version(void);
+
+/* ---- input-script surface --------------------------------------------- */
+
+/* Execute one input-script line.  lammps_command (src/library.h:31) ->
+ * Input::execute_command (src/input.cpp:542-579).  Accepted: units lj,
+ * dimension 3, boundary p p p, atom_style mdpd/atomic/meso <rho_est> |
+ * dpd/atomic/meso, mass, group ID type|id ..., pair_style mdpd/meso T rc seed,
+ * pair_style dpd/meso rc seed, pair_coeff, fix ID G nve/meso,
+ * fix ID G aboundary/rho/meso cut_rho rd [weight quadratic|lucy],
+ * fix ID G addforce/meso fx fy fz, unfix, timestep, neighbor skin bin,
+ * neigh_modify every/delay/check, run_style mvv/meso, reset_timestep,
+ * thermo, run N.  Comments (#) and blank lines are
+ * ignored; echo/log/print/newton/comm_modify/atom_modify are accepted and
+ * ignored. */
+int mdpd_command(mdpd_system *sys, const char *line);
+/* Execute every line of a text buffer.  lammps_file (src/library.h:30) */
+int mdpd_commands_string(mdpd_system *sys, const char *text);
+
+/* ---- box and atoms ------------------------------------------------------ */
+
+/* create_box: orthogonal periodic box with ntypes atom types */
+int mdpd_create_box(mdpd_system *sys, int ntypes, const double lo[3], const double hi[3]);
+/* Append n atoms: tag (NULL: next ids), type 1..ntypes, mask (NULL: group
+ * "all" only), x and v as n*3 doubles (v may be NULL).  Positions and
+ * velocities are stored in FP32 on the device.  Atoms are only accepted
+ * before the first setup/run, or after mdpd_clear_atoms. */
+int mdpd_create_atoms(mdpd_system *sys, int n, const int *tag, const int *type,
+                      const int *mask, const double *x, const double *v);
+int mdpd_clear_atoms(mdpd_system *sys);
+/* lammps_get_natoms (src/library.h:40) */
+int mdpd_get_natoms(const mdpd_system *sys);
+/* bitmask of a group name (0 if unknown) */
+int mdpd_group_bit(const mdpd_system *sys, const char *name);
+
+/* ---- run ---------------------------------------------------------------- */
+
+/* Verlet setup: wrap, sort, neighbor build, density and forces of the
+ * current state (ModifiedVerlet::setup, src/USER-MESO/mvv_meso.cu:203-283). */
+int mdpd_setup(mdpd_system *sys);
+/* run N (ModifiedVerlet::run, src/USER-MESO/mvv_meso.cu:309-572); performs
+ * setup first if needed.  Rebuilds follow neigh_modify every/delay/check
+ * with an on-device displacement test. */
+int mdpd_run(mdpd_system *sys, long long nsteps);
+long long mdpd_get_timestep(const mdpd_system *sys);
+/* number of neighbor-list builds so far */
+long long mdpd_get_nbuild(const mdpd_system *sys);
+
+/* ---- results ------------------------------------------------------------- */
+
+/* lammps_gather_atoms (src/library.h:41): copy a per-atom quantity in the
+ * order atoms were created.  type 0 = int, 1 = double; count values per atom.
+ * Names: "x" (1,3) "v" (1,3) "f" (1,3) "rho" (1,1) "e_pair" (1,1)
+ * "virial" (1,6: xx yy zz xy xz yz) "tag" (0,1) "type" (0,1) "mask" (0,1)
+ * "image" (0,3).  f, e_pair and virial are those of the last force pass of
+ * setup or of the last step of a run. */
+int mdpd_gather_atoms(mdpd_system *sys, const char *name, int type, int count, void *data);
+/* lammps_scatter_atoms (src/library.h:42) for "x" and "v" (type 1, count 3):
+ * overwrite positions / velocities in creation order; the next run repeats
+ * setup. */
+int mdpd_scatter_atoms(mdpd_system *sys, const char *name, int type, int count, const void *data);
+/* thermo of the last force pass: out[0] temperature (compute temp/meso,
+ * dof = 3N-3), out[1] pressure (compute pressure/meso), out[2] pair energy,
+ * out[3] kinetic energy.  compute_temp_meso.cu:49-101,
+ * compute_pressure_meso.cu:115-155 */
+int mdpd_thermo(mdpd_system *sys, double out[4]);
+
+/* ---- parity hooks: the engine's internal tables --------------------------
+ * These expose what the reference keeps in meso_atom / MesoNeighList device
+ * arrays (atom_meso.cu:271-387, neigh_build_meso.cu:17-127), in the engine's
+ * sorted order, for bit-exact comparison with the formulas. */
+
+/* Sorted order of the last build: origin[k] = creation index of the particle
+ * in sorted slot k; key[k] = its 2-level Morton sort key (border bit 0);
+ * xw = its wrapped FP32 position at that build (3 per slot).  Any pointer may
+ * be NULL. */
+int mdpd_debug_sorted(mdpd_system *sys, int *origin, unsigned long long *key, float *xw);
+/* Neighbor rows in sorted order: counts[k], rows[k*maxnbr + p] (entry =
+ * sorted index j | image << 27, image = (sx+1)+3(sy+1)+9(sz+1)).  *maxnbr
+ * receives the row width; rows may be NULL to query it. */
+int mdpd_debug_neighbors(mdpd_system *sys, int *maxnbr, int *counts, int *rows);
+/* TEA signatures of the next force pass, in creation order. */
+int mdpd_debug_signatures(mdpd_system *sys, unsigned int *sig);
+
+/* Stateless device evaluation (no handle): n pair Gaussians
+ * gaussian_TEA<4>(a > b, a, b) (math_meso.h:465-473) in FP32, and n
+ * premix_TEA<rounds>(a, b) values (math_meso.h:459-463). */
+int mdpd_tea_gaussian(int device, int n, const unsigned int *a, const unsigned int *b, float *out);
+int mdpd_tea_premix(int device, int n, const unsigned int *a, const unsigned int *b, int rounds,
+                    unsigned int *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,127 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+declared in include/*.h, and the input-script surface validates commands the
+way the reference styles do (error paths return codes with messages)."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+
+
+def declared_functions():
+    names = []
+    for h in glob.glob(os.path.join(REPO, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(mdpd_[a-z_0-9]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_exports_every_declared_symbol(engine):
+    lib = ctypes.CDLL(engine.library_path())
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_version(engine):
+    assert b"mdpd355" in engine.load().mdpd_version()
+
+
+def _deck(s, L=8.0):
+    s.commands("""
+        units lj
+        dimension 3
+        boundary p p p
+        atom_style mdpd/atomic/meso 6.05
+    """)
+    s.create_box(1, (0, 0, 0), (L, L, L))
+
+
+def test_script_surface_accepts_reference_deck(engine):
+    s = engine.System()
+    _deck(s)
+    s.commands("""
+        neighbor 0.3 bin
+        neigh_modify every 1 delay 0 check yes
+        mass 1 1.0
+        pair_style mdpd/meso 1.0 1.0 123456
+        pair_coeff * * -40 25 4.5 1.0 0.75       # cutA=1.0, cutB=0.75
+        fix BC all aboundary/rho/meso cut_rho 0.75
+        timestep 0.01
+        fix int_f all nve/meso
+        run_style mvv/meso
+        thermo 100
+    """)
+    s.close()
+
+
+@pytest.mark.parametrize("line,code", [
+    ("pair_coeff * * -40 25 4.5 0.7 0.75", 1),            # cutA < cutB (pair_mdpd_meso.cu:346)
+    ("pair_coeff * * -40 25 4.5 1.0", 1),                 # wrong arity (:332)
+    ("pair_coeff 2 2 -40 25 4.5 1.0 0.75", 1),            # type out of range
+    ("pair_style mdpd/meso 1.0 1.0", 1),                  # too few args (:301)
+    ("fix r all aboundary/rho/meso", 1),                  # no cut_rho (:98-99)
+    ("fix r nogroup nve/meso", 1),                        # unknown group
+    ("pair_style lj/cut 2.5", 6),                         # outside this engine
+    ("run_style verlet", 6),
+    ("frobnicate 1 2", 6),
+])
+def test_script_errors(engine, line, code):
+    s = engine.System()
+    _deck(s)
+    s.command("pair_style mdpd/meso 1.0 1.0 123456")
+    with pytest.raises(engine.MdpdError) as ei:
+        s.command(line)
+    assert ei.value.code == code
+    assert str(ei.value)
+    s.close()
+
+
+def test_atom_style_requires_density_estimate(engine):
+    s = engine.System()
+    with pytest.raises(engine.MdpdError) as ei:
+        s.command("atom_style mdpd/atomic/meso")
+    assert "estimation" in str(ei.value)
+
+
+def test_pair_coeff_before_style_and_missing_coeffs(engine):
+    s = engine.System()
+    _deck(s)
+    with pytest.raises(engine.MdpdError) as ei:
+        s.command("pair_coeff * * -40 25 4.5 1.0 0.75")
+    assert ei.value.code == 2
+    x = np.random.default_rng(0).uniform(0, 8, (100, 3))
+    s.create_atoms(np.ones(100, np.int32), x)
+    s.commands("mass 1 1.0\npair_style mdpd/meso 1.0 1.0 5\nrun_style mvv/meso")
+    with pytest.raises(engine.MdpdError) as ei:
+        s.setup()
+    assert "pair coeffs" in str(ei.value)
+
+
+def test_groups_and_gather_before_setup(engine):
+    s = engine.System()
+    s.command("atom_style dpd/atomic/meso")
+    s.create_box(2, (0, 0, 0), (5, 5, 5))
+    x = np.random.default_rng(1).uniform(0, 5, (10, 3))
+    t = np.array([1, 2] * 5, np.int32)
+    s.create_atoms(t, x)
+    s.command("group walls type 2")
+    bit = s.group_bit("walls")
+    assert bit == 2
+    m = s.gather("mask")
+    assert np.array_equal(m, np.where(t == 2, 1 | bit, 1))
+    assert np.allclose(s.gather("x"), x)
+    assert np.array_equal(s.gather("tag"), np.arange(1, 11))
+
+
+def test_open_does_not_need_a_device(engine):
+    # parsing works without touching HIP; setup is where the device is needed
+    s = engine.System(device=0)
+    s.command("units lj")
+    s.close()

@@ -1,0 +1,233 @@
+// Shared definitions for the mdpd355 engine: the per-system descriptor the
+// kernels read, particle-array bundles, and the device-side integer/float
+// helpers (TEA, signatures, Morton interleave, the FP32 pair Gaussian).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdpd {
+
+constexpr int WAVE = 64;
+constexpr int MAXTYPES = 8;
+constexpr int NCOEF = 10;          // per type pair, see Coef
+constexpr int IMG_SHIFT = 27;      // neighbor entry = j | image << 27
+constexpr uint32_t IDX_MASK = (1u << IMG_SHIFT) - 1u;
+constexpr int IMG_ZERO = 13;       // (0,0,0) image: (sx+1) + 3(sy+1) + 9(sz+1)
+
+enum Style : int { STYLE_MDPD = 0, STYLE_DPD = 1 };
+
+// Per type pair coefficients in FP32, laid out as NCOEF floats.
+// mDPD (pair_mdpd_meso.cu:70-91): rc, 1/rc, rd, 1/rd, A, B, gamma, sigma.
+// DPD  (pair_dpd_meso.cu):        rc, 1/rc, -, -,   a0, -, gamma, sigma, expw.
+enum Coef : int { C_RC = 0, C_RCINV, C_RD, C_RDINV, C_A, C_B, C_GAMMA, C_SIGMA, C_EXPW, C_RC2 };
+
+// One simulated box.  Static fields are written by the host before a
+// launch; dynamic fields are written only by the commit points named below.
+struct SysDev {
+    // geometry (FP32 copies for the per-step kernels)
+    float lo[3], hi[3], L[3];
+    // bin geometry in FP64 for the sort-key formula (neighbor_meso.cu:837-853)
+    double lo_d[3], binsize_d[3], bininv_d[3];
+    int mbin[3];                 // bins incl. ghost layer
+    int nb[3];                   // interior cells per dim (= mbin - 2)
+    int ncell;                   // nb[0]*nb[1]*nb[2]
+    int cell_base;               // first cell slot in pooled cell arrays (ncell+1 slots)
+    int part_base;               // first particle slot (multiple of 64)
+    int n;                       // particles
+    int npad;                    // n rounded up to 64
+    int l1_shift;                // 12
+    float rc2_list;              // (float)((rc+skin)^2)
+    float trig2;                 // (skin/2)^2
+    float dt, dtf, dtinvsqrt;
+    uint32_t seed;
+    int style;
+    int ntypes;
+    int coef_base;               // offset (in floats) into the pooled coefficient table
+    int mass_base;               // offset into the pooled per-type 1/m, m tables
+    int nve_bit, rho_bit, add_bit;
+    float add[3];
+    float rd, rho_factor;
+    int lucy;
+    int maxnbr;                  // neighbor row width for this pool
+    // dynamic state
+    int sel;                     // buffer holding x, v (flips every step and at re-sort)
+    int sela;                    // buffer holding tag, mask, origin, image (flips at re-sort)
+    int stale;                   // neighbor list must be rebuilt before the next force pass
+    int stale_pending;           // set during a step by the displacement check
+    int done;                    // last-wave commit counter
+    int overflow;                // a neighbor row or cell exceeded its capacity
+    int active;                  // 0: slot idle (batched / served engine)
+    int steps_left;              // served engine: steps still to run
+    long long step;              // ntimestep of the next / current force evaluation
+    uint32_t seed_next;          // premix_TEA<64>(seed, step + 1)
+    int pad_;
+};
+
+// Device arrays of one pool (a pool packs one or more systems).
+struct PoolDev {
+    float4 *pos[2];      // x, y, z, bits(type-1)
+    float4 *vel[2];      // vx, vy, vz, bits(signature)
+    int *tag[2], *mask[2], *origin[2], *image[2];
+    float4 *xhold;       // positions at the last build
+    float *rho;
+    float4 *frc;         // fx, fy, fz, e_pair   (output steps only)
+    float *vir;          // 6 * cap, SoA blocks  (output steps only)
+    float4 *vend;        // end-of-step velocity  (output steps only)
+    // sort / cell scratch
+    int *pcell;          // global cell slot per particle
+    int *pslot;          // arrival rank within the cell
+    int *psub;           // 12-bit sub-bin Morton code
+    int *tmp;            // particle index per sorted slot (pre-order)
+    unsigned long long *key;  // optional: reference sort key per particle (debug)
+    int *cell_count;     // per cell slot
+    int *cell_start;     // per cell slot, global particle index; slot ncell = end
+    int *cell_xyz;       // packed interior coords of the cell at this Morton rank
+    int *cell_rank;      // Morton rank of linear cell id (per system, cell_base + lin)
+    int *cell_sys;       // system of each cell slot
+    int *nlist;          // (cap/64) tiles * maxnbr * 64
+    int *ncount;         // per particle
+    int *wave_sys;       // system of each 64-particle wave
+    SysDev *sys;
+    float *coef;         // pooled coefficient tables
+    float *minv;         // pooled per-type 1/m
+    float *mass;         // pooled per-type m
+    int cap;             // particle slots (multiple of 64)
+    int ncellslots;
+    int nsys;
+    int maxnbr;
+};
+
+// ----------------------------------------------------------------------------
+// TEA and friends: src/USER-MESO/math_meso.h:178-201,435-463
+// ----------------------------------------------------------------------------
+
+__host__ __device__ inline void tea_rounds(uint32_t &v0, uint32_t &v1, int rounds)
+{
+    uint32_t sum = 0;
+    for (int r = 0; r < rounds; ++r) {
+        sum += 0x9E3779B9u;
+        v0 += ((v1 << 4) + 0xA341316Cu) ^ (v1 + sum) ^ ((v1 >> 5) + 0xC8013EA4u);
+        v1 += ((v0 << 4) + 0xAD90777Du) ^ (v0 + sum) ^ ((v0 >> 5) + 0x7E95761Eu);
+    }
+}
+
+template <int N>
+__host__ __device__ inline uint32_t premix(uint32_t v0, uint32_t v1)
+{
+    uint32_t sum = 0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        sum += 0x9E3779B9u;
+        v0 += ((v1 << 4) + 0xA341316Cu) ^ (v1 + sum) ^ ((v1 >> 5) + 0xC8013EA4u);
+        v1 += ((v0 << 4) + 0xAD90777Du) ^ (v0 + sum) ^ ((v0 >> 5) + 0x7E95761Eu);
+    }
+    return v0 ^ v1;
+}
+
+__host__ inline uint32_t premix_host(uint32_t v0, uint32_t v1, int rounds)
+{
+    tea_rounds(v0, v1, rounds);
+    return v0 ^ v1;
+}
+
+__device__ inline uint32_t brev32(uint32_t x) { return __builtin_bitreverse32(x); }
+
+// bit i of x -> bit 3i (x < 2^10 here: coordinates and 11-bit mantissa slices)
+__host__ __device__ inline uint64_t spread3(uint32_t x)
+{
+    uint64_t v = x & 0x1FFFFFu;
+    v = (v | (v << 32)) & 0x1F00000000FFFFull;
+    v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+    v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+    v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__host__ __device__ inline uint64_t morton3(uint32_t i, uint32_t j, uint32_t k)
+{
+    return spread3(i) | (spread3(j) << 1) | (spread3(k) << 2);
+}
+
+__device__ inline uint32_t mantissa_bits(float u, float v, float w)
+{
+    uint32_t i = (__float_as_uint(u) & 0x7FF000u) >> 12;
+    uint32_t j = (__float_as_uint(v) & 0x7FF000u) >> 12;
+    uint32_t k = (__float_as_uint(w) & 0x7FF000u) >> 12;
+    return (uint32_t)morton3(i, j, k);
+}
+
+// mDPD signature: atom_vec_mdpd_atomic_meso.cu:166
+__device__ inline uint32_t sig_mdpd(int tag, uint32_t seed_t) { return premix<32>(brev32((uint32_t)tag), seed_t); }
+// DPD signature: atom_vec_meso.cu:164
+__device__ inline uint32_t sig_dpd(int tag, uint32_t seed_t, float vx, float vy, float vz)
+{
+    return seed_t ^ premix<16>(brev32((uint32_t)tag), mantissa_bits(vx, vy, vz));
+}
+
+// ----------------------------------------------------------------------------
+// FP32 pair Gaussian from the four-round TEA bits (math_meso.h:465-473).
+// cos(pi x) and ln u are evaluated in FP32 from the mathematics:
+//  * cos(pi x) = -sin(pi t), t = x - 1/2 in [-1/2, 1/2), with the Taylor
+//    series of sin to t^11 (truncation < 6e-8);
+//  * ln u for u = v1 / 2^32: for v1 >= 2^31 through log1p of the exact
+//    complement (Kahan's log1p identity, so u near 1 keeps its precision),
+//    otherwise log2 - 32 (|log2 u| >= 1, no cancellation).
+// ----------------------------------------------------------------------------
+
+__device__ inline float sinpi_half(float t)
+{
+    // sin(pi t), |t| <= 1/2; c_k = (-1)^k pi^(2k+1) / (2k+1)!
+    const float t2 = t * t;
+    float s = -0.0073704309457143504f;        // -pi^11/11!
+    s = fmaf(s, t2, 0.082145886611128228f);   //  pi^9/9!
+    s = fmaf(s, t2, -0.59926452932079207f);   // -pi^7/7!
+    s = fmaf(s, t2, 2.5501640398773455f);     //  pi^5/5!
+    s = fmaf(s, t2, -5.1677127800499703f);    // -pi^3/3!
+    s = fmaf(s, t2, 3.1415926535897932f);     //  pi
+    return s * t;
+}
+
+__device__ inline float log1p_kahan(float y)
+{
+    float w = 1.0f + y;
+    if (w == 1.0f) return y;
+    return __logf(w) * (y / (w - 1.0f));
+}
+
+__device__ inline float pair_gaussian(uint32_t si, uint32_t sj)
+{
+    uint32_t v0 = si > sj ? si : sj;
+    uint32_t v1 = si > sj ? sj : si;
+    tea_rounds(v0, v1, 4);
+    // x = (v0 & 0x7FFFFFFF) 2^-31 ; t = x - 1/2 exactly from the integer
+    const int ti = (int)(v0 & 0x7FFFFFFFu) - (1 << 30);
+    const float t = (float)ti * 4.656612873077392578125e-10f;
+    float f = -sinpi_half(t);
+    f = (v0 & 0x80000000u) ? f : -f;
+    const uint32_t w = v1 > 1u ? v1 : 1u;
+    float lnu;
+    if (w >= 0x80000000u) {
+        const float y = -(float)(0u - w) * 2.3283064365386962890625e-10f;
+        lnu = log1p_kahan(y);
+    } else {
+        lnu = (__log2f((float)w) - 32.0f) * 0.69314718055994530942f;
+    }
+    const float r = __fsqrt_rn(-2.0f * lnu);
+    float g = r * f;
+    return fminf(fmaxf(g, -4.0f), 4.0f);
+}
+
+// image code -> shift multipliers
+__device__ inline void image_decode(uint32_t code, int &sx, int &sy, int &sz)
+{
+    int c = (int)code;
+    sz = (c * 57) >> 9;          // c / 9 for c < 27
+    int r = c - 9 * sz;
+    sy = (r * 11) >> 5;          // r / 3 for r < 9
+    sx = r - 3 * sy;
+    sx -= 1; sy -= 1; sz -= 1;
+}
+
+}  // namespace mdpd

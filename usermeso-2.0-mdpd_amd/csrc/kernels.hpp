@@ -1,0 +1,30 @@
+// Host-side launchers for the mdpd355 kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace mdpd {
+
+enum ForceMode : int {
+    MODE_SETUP = 0,      // forces (+energy/virial) of the current state, no state change
+    MODE_FUSED = 1,      // force, final half-kick, next initial half-kick + drift
+    MODE_FUSED_OUT = 2,  // MODE_FUSED and also write f, e, virial, end-of-step v
+    MODE_FINAL = 3,      // force and final half-kick only (end of a run)
+};
+
+// rebuild phase: wrap, keys, sort, gather, neighbor rows (stale systems only)
+hipError_t launch_rebuild(const PoolDev &P, hipStream_t st);
+hipError_t launch_nlist_only(const PoolDev &P, hipStream_t st);
+hipError_t launch_commit_rebuild(const PoolDev &P, hipStream_t st);
+hipError_t launch_signatures(const PoolDev &P, hipStream_t st);
+
+// per step
+hipError_t launch_density(const PoolDev &P, hipStream_t st);
+hipError_t launch_initial(const PoolDev &P, hipStream_t st);
+hipError_t launch_force(const PoolDev &P, int style, int mode, bool ev, hipStream_t st);
+
+// stateless helpers for parity tests
+hipError_t launch_pair_gaussian(const uint32_t *a, const uint32_t *b, float *out, int n, hipStream_t st);
+hipError_t launch_premix(const uint32_t *a, const uint32_t *b, uint32_t *out, int n, int rounds, hipStream_t st);
+
+}  // namespace mdpd

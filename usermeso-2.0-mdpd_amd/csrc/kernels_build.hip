@@ -1,0 +1,379 @@
+// Neighbor-rebuild phase of the mdpd355 engine: periodic wrap, cell keys,
+// counting sort by cell + LDS per-cell sort of the sub-bin key, gather into the
+// other buffer, wave64 ballot/prefix neighbor-list build, commit.
+//
+// Replaces, for one or many periodic boxes at once:
+//   MesoAtom::sort_local            src/USER-MESO/atom_meso.cu:271-387
+//   MesoNeighbor::binning_meso      src/USER-MESO/neighbor_meso.cu:390-683
+//   MesoNeighbor::full_bin_meso     src/USER-MESO/neigh_build_meso.cu:17-127,286-449
+//   Domain::pbc + MesoComm::borders (on-device periodic images instead of ghosts)
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace mdpd {
+
+#pragma clang fp contract(off)
+
+// --------------------------------------------------------------------------
+// K1: wrap into the box (Domain::pbc in FP32), reference sort key, cell slot,
+// arrival rank inside the cell.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_keys(PoolDev P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.cap) return;
+    const int s = P.wave_sys[i >> 6];
+    if (s < 0) return;
+    SysDev *d = P.sys + s;
+    if (!d->active || !d->stale) return;
+    if (i >= d->part_base + d->n) return;
+
+    const int cur = d->sel, cura = d->sela;
+    float4 p = P.pos[cur][i];
+    int img = P.image[cura][i];
+    float c[3] = {p.x, p.y, p.z};
+    int im[3] = {(img & 1023) - 512, ((img >> 10) & 1023) - 512, ((img >> 20) & 1023) - 512};
+    for (int a = 0; a < 3; ++a) {
+        const float flo = d->lo[a], fhi = d->hi[a], prd = d->L[a];
+        float v = c[a];
+        if (v < flo) { v = v + prd; im[a] -= 1; }
+        if (v >= fhi) { v = v - prd; v = fmaxf(v, flo); im[a] += 1; }
+        c[a] = v;
+    }
+    P.pos[cur][i] = make_float4(c[0], c[1], c[2], p.w);
+    P.image[cura][i] = ((im[0] + 512) & 1023) | (((im[1] + 512) & 1023) << 10) | (((im[2] + 512) & 1023) << 20);
+
+    int b[3];
+    uint32_t sub[3];
+    for (int a = 0; a < 3; ++a) {
+        // bin: (int)((x - lo) * bininv + 1.0), FP64, clamped to the interior
+        const double t = ((double)c[a] - d->lo_d[a]) * d->bininv_d[a];
+        const double u = t + 1.0;
+        int bb = (int)u;
+        bb = bb > d->mbin[a] - 2 ? d->mbin[a] - 2 : bb;
+        bb = bb < 1 ? 1 : bb;
+        b[a] = bb;
+        // sub-bin: (x - (bin-1) * binsize) * 16 * bininv (atom_meso.cu:298-300)
+        const double off = (double)(uint32_t)(bb - 1) * d->binsize_d[a];
+        const double rel = (double)c[a] - off;
+        const double sc = rel * (16.0 * d->bininv_d[a]);
+        int ss = (int)sc;
+        ss = ss > 15 ? 15 : ss;
+        ss = ss < 0 ? 0 : ss;
+        sub[a] = (uint32_t)ss;
+    }
+    const uint32_t z2 = (uint32_t)morton3(sub[0], sub[1], sub[2]);
+    if (P.key) {
+        const uint64_t z1 = morton3((uint32_t)b[0], (uint32_t)b[1], (uint32_t)b[2]);
+        P.key[i] = (z1 << d->l1_shift) | z2;
+    }
+    const int lin = (b[0] - 1) + d->nb[0] * ((b[1] - 1) + d->nb[1] * (b[2] - 1));
+    const int gc = d->cell_base + P.cell_rank[d->cell_base + lin];
+    P.pcell[i] = gc;
+    P.psub[i] = (int)z2;
+    P.pslot[i] = atomicAdd(P.cell_count + gc, 1);
+}
+
+// --------------------------------------------------------------------------
+// K2: exclusive scan of cell counts in Morton-rank order, one workgroup per
+// system; clears the counts for the next rebuild.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_scan(PoolDev P)
+{
+    const int s = blockIdx.x;
+    SysDev *d = P.sys + s;
+    if (!d->active || !d->stale) return;
+    __shared__ int part[1024];
+    const int nc = d->ncell, base = d->cell_base;
+    const int per = (nc + 1023) / 1024;
+    const int lo = threadIdx.x * per;
+    const int hi = min(lo + per, nc);
+    int sum = 0;
+    for (int k = lo; k < hi; ++k) sum += P.cell_count[base + k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 partial sums
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = d->part_base + part[threadIdx.x] - sum;
+    for (int k = lo; k < hi; ++k) {
+        const int cnt = P.cell_count[base + k];
+        P.cell_start[base + k] = run;
+        P.cell_count[base + k] = 0;
+        run += cnt;
+    }
+    if (threadIdx.x == 0) P.cell_start[base + nc] = d->part_base + d->n;
+}
+
+// --------------------------------------------------------------------------
+// K3: scatter particle indices into their cell segments (arrival order).
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_scatter(PoolDev P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.cap) return;
+    const int s = P.wave_sys[i >> 6];
+    if (s < 0) return;
+    SysDev *d = P.sys + s;
+    if (!d->active || !d->stale) return;
+    if (i >= d->part_base + d->n) return;
+    P.tmp[P.cell_start[P.pcell[i]] + P.pslot[i]] = i;
+}
+
+// --------------------------------------------------------------------------
+// K4: one wave per cell: order the cell's particles by (sub-bin key, previous
+// index) -- the stable sort of the 2-level Morton key -- and gather every
+// per-particle array into the other buffer.
+// --------------------------------------------------------------------------
+constexpr int K4_LDS = 128;
+
+__global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
+{
+    __shared__ int s_key[4][K4_LDS];
+    __shared__ int s_idx[4][K4_LDS];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int slot = blockIdx.x * 4 + wv;
+    if (slot >= P.ncellslots) return;
+    const int s = P.cell_sys[slot];
+    if (s < 0) return;
+    SysDev *d = P.sys + s;
+    if (!d->active || !d->stale) return;
+    if (slot - d->cell_base >= d->ncell) return;   // end sentinel
+    const int start = P.cell_start[slot], end = P.cell_start[slot + 1];
+    const int m = end - start;
+    if (m <= 0) return;
+    const int cur = d->sel, cura = d->sela;
+    const bool in_lds = m <= K4_LDS;
+    if (in_lds) {
+        for (int e = lane; e < m; e += WAVE) {
+            const int idx = P.tmp[start + e];
+            s_idx[wv][e] = idx;
+            s_key[wv][e] = P.psub[idx];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int e = lane; e < m; e += WAVE) {
+        const int idx = in_lds ? s_idx[wv][e] : P.tmp[start + e];
+        const int key = in_lds ? s_key[wv][e] : P.psub[idx];
+        int rank = 0;
+        for (int f = 0; f < m; ++f) {
+            const int kf = in_lds ? s_key[wv][f] : P.psub[P.tmp[start + f]];
+            const int jf = in_lds ? s_idx[wv][f] : P.tmp[start + f];
+            rank += (kf < key) || (kf == key && jf < idx);
+        }
+        const int dst = start + rank;
+        const float4 p = P.pos[cur][idx];
+        P.pos[cur ^ 1][dst] = p;
+        P.vel[cur ^ 1][dst] = P.vel[cur][idx];
+        P.xhold[dst] = p;
+        P.tag[cura ^ 1][dst] = P.tag[cura][idx];
+        P.mask[cura ^ 1][dst] = P.mask[cura][idx];
+        P.origin[cura ^ 1][dst] = P.origin[cura][idx];
+        P.image[cura ^ 1][dst] = P.image[cura][idx];
+        if (P.key) P.key[(size_t)P.cap + dst] = P.key[idx];
+    }
+}
+
+// --------------------------------------------------------------------------
+// K5: one wave per cell: neighbor rows for the cell's particles.  Lanes take
+// candidates from the 27 stencil cells (periodic images), every centre is
+// tested with one ballot, and hits are appended at count + mbcnt(ballot) --
+// the wave64 form of gpu_build_neighbor_list (neigh_build_meso.cu:58-118).
+// Hit test: (float) r^2 <= (float)(rc+skin)^2 without contraction
+// (neigh_build_meso.cu:86-93).  Row layout: tile (i/64), entry p at
+// tile*maxnbr*64 + p*64 + i%64.
+// --------------------------------------------------------------------------
+constexpr int K5_CENTRES = 128;
+
+__global__ void __launch_bounds__(256) k_nlist(PoolDev P)
+{
+    __shared__ float4 s_ci[4][K5_CENTRES];
+    __shared__ int s_cnt[4][K5_CENTRES];
+    __shared__ int s_start[4][32], s_pre[4][32], s_code[4][32];
+    __shared__ float s_sh[4][32][3];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int slot = blockIdx.x * 4 + wv;
+    if (slot >= P.ncellslots) return;
+    const int s = P.cell_sys[slot];
+    if (s < 0) return;
+    SysDev *d = P.sys + s;
+    if (!d->active || !d->stale) return;
+    const int rel = slot - d->cell_base;
+    if (rel >= d->ncell) return;
+    const int start = P.cell_start[slot], end = P.cell_start[slot + 1];
+    const int m = end - start;
+    if (m <= 0) return;
+    const int nxt = d->sel ^ 1;      // sorted data written by k_sortgather
+    const float4 *pos = P.pos[nxt];
+    const int maxnbr = P.maxnbr;
+
+    // stencil: lane k < 27 describes neighbor cell k
+    const int xyz = P.cell_xyz[slot];
+    const int cx = xyz & 1023, cy = (xyz >> 10) & 1023, cz = (xyz >> 20) & 1023;
+    int len = 0;
+    if (lane < 27) {
+        const int ox = lane % 3 - 1, oy = (lane / 3) % 3 - 1, oz = lane / 9 - 1;
+        int nx = cx + ox, ny = cy + oy, nz = cz + oz, sx = 0, sy = 0, sz = 0;
+        if (nx < 0) { nx += d->nb[0]; sx = -1; } else if (nx >= d->nb[0]) { nx -= d->nb[0]; sx = 1; }
+        if (ny < 0) { ny += d->nb[1]; sy = -1; } else if (ny >= d->nb[1]) { ny -= d->nb[1]; sy = 1; }
+        if (nz < 0) { nz += d->nb[2]; sz = -1; } else if (nz >= d->nb[2]) { nz -= d->nb[2]; sz = 1; }
+        const int lin = nx + d->nb[0] * (ny + d->nb[1] * nz);
+        const int ns = d->cell_base + P.cell_rank[d->cell_base + lin];
+        const int st = P.cell_start[ns];
+        len = P.cell_start[ns + 1] - st;
+        s_start[wv][lane] = st;
+        s_code[wv][lane] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+        s_sh[wv][lane][0] = (float)sx * d->L[0];
+        s_sh[wv][lane][1] = (float)sy * d->L[1];
+        s_sh[wv][lane][2] = (float)sz * d->L[2];
+    }
+    // exclusive prefix of the 27 lengths
+    int incl = len;
+    for (int off = 1; off < 32; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane < 27) s_pre[wv][lane] = incl - len;
+    const int total = __shfl(incl, 26, 64);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+
+    const float rc2 = d->rc2_list;
+    int any_over = 0;
+    for (int c0 = 0; c0 < m; c0 += K5_CENTRES) {
+        const int mc = min(K5_CENTRES, m - c0);
+        for (int c = lane; c < mc; c += WAVE) {
+            s_ci[wv][c] = pos[start + c0 + c];
+            s_cnt[wv][c] = 0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int b0 = 0; b0 < total; b0 += WAVE) {
+            const int mi = b0 + lane;
+            const bool valid = mi < total;
+            int k = 0;
+            if (valid) {
+                // largest k with pre[k] <= mi (pre is non-decreasing)
+                int lo = 0, hi = 26;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[wv][mid] <= mi) lo = mid; else hi = mid - 1;
+                }
+                k = lo;
+            }
+            int j = 0, code = IMG_ZERO;
+            float xj = 0.f, yj = 0.f, zj = 0.f;
+            if (valid) {
+                j = s_start[wv][k] + (mi - s_pre[wv][k]);
+                code = s_code[wv][k];
+                const float4 pj = pos[j];
+                xj = pj.x + s_sh[wv][k][0];
+                yj = pj.y + s_sh[wv][k][1];
+                zj = pj.z + s_sh[wv][k][2];
+            }
+            const uint32_t entry = (uint32_t)j | ((uint32_t)code << IMG_SHIFT);
+            for (int c = 0; c < mc; ++c) {
+                const float4 ci = s_ci[wv][c];
+                const float dx = ci.x - xj, dy = ci.y - yj, dz = ci.z - zj;
+                const float r2 = (dx * dx + dy * dy) + dz * dz;
+                const int ic = start + c0 + c;
+                const bool hit = valid && r2 <= rc2 && !(j == ic && code == IMG_ZERO);
+                const uint64_t bal = __ballot(hit);
+                if (bal) {
+                    const int n0 = s_cnt[wv][c];
+                    const int p = n0 + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    if (hit && p < maxnbr)
+                        P.nlist[(size_t)(ic >> 6) * maxnbr * WAVE + (size_t)p * WAVE + (ic & 63)] = (int)entry;
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) s_cnt[wv][c] = n0 + __popcll(bal);
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+        }
+        for (int c = lane; c < mc; c += WAVE) {
+            const int n = s_cnt[wv][c];
+            P.ncount[start + c0 + c] = n;
+            any_over |= n > maxnbr;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (__ballot(any_over) && lane == 0) atomicOr(&d->overflow, 1);
+}
+
+// --------------------------------------------------------------------------
+// Commit after a rebuild: the sorted data in the other buffers becomes current.
+// --------------------------------------------------------------------------
+__global__ void k_commit_rebuild(PoolDev P)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.nsys) return;
+    SysDev *d = P.sys + s;
+    if (!d->active || !d->stale) return;
+    d->sel ^= 1;
+    d->sela ^= 1;
+    d->stale = 0;
+}
+
+// --------------------------------------------------------------------------
+// Signatures of the current step (setup only; in the step loop the fused
+// force epilogue writes them for the following step).
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_signatures(PoolDev P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.cap) return;
+    const int s = P.wave_sys[i >> 6];
+    if (s < 0) return;
+    SysDev *d = P.sys + s;
+    if (!d->active || i >= d->part_base + d->n) return;
+    const uint32_t st = premix<64>(d->seed, (uint32_t)d->step);
+    float4 v = P.vel[d->sel][i];
+    const int tag = P.tag[d->sela][i];
+    const uint32_t sg = d->style == STYLE_MDPD ? sig_mdpd(tag, st) : sig_dpd(tag, st, v.x, v.y, v.z);
+    v.w = __uint_as_float(sg);
+    P.vel[d->sel][i] = v;
+}
+
+// ----------------------------------------------------------------------------
+// host launchers
+// ----------------------------------------------------------------------------
+static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+hipError_t launch_rebuild(const PoolDev &P, hipStream_t st)
+{
+    if (P.cap == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_keys, dim3(nblk(P.cap, 256)), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(k_scan, dim3(P.nsys), dim3(1024), 0, st, P);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(P.cap, 256)), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(k_sortgather, dim3(nblk(P.ncellslots, 4)), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(k_nlist, dim3(nblk(P.ncellslots, 4)), dim3(256), 0, st, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_nlist_only(const PoolDev &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_nlist, dim3(nblk(P.ncellslots, 4)), dim3(256), 0, st, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit_rebuild(const PoolDev &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_commit_rebuild, dim3(nblk(P.nsys, 64)), dim3(64), 0, st, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_signatures(const PoolDev &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_signatures, dim3(nblk(P.cap, 256)), dim3(256), 0, st, P);
+    return hipGetLastError();
+}
+
+}  // namespace mdpd
