@@ -109,6 +109,16 @@ int mdpd_scatter_atoms(mdpd_system *sys, const char *name, int type, int count, 
  * compute_pressure_meso.cu:115-155 */
 int mdpd_thermo(mdpd_system *sys, double out[4]);
 
+/* ---- kernel timing ---------------------------------------------------------
+ * HIP events recorded around every launch on the handle's stream, folded into
+ * per-phase totals at the end of each setup / run.  Phases: "rebuild"
+ * (wrap, keys, sort, gather, neighbor rows), "density", "force" (pair force
+ * with the fused integration), "initial" (first half-kick of a run).  The
+ * reference's counterpart is MesoTimer::device_stamp and the -profile switch
+ * (src/USER-MESO/timer_meso.cu:14-19, src/lammps.cpp:189-191). */
+int mdpd_profile(mdpd_system *sys, int enable);          /* also resets the totals */
+int mdpd_profile_get(mdpd_system *sys, const char *phase, double *total_ms, long long *launches);
+
 /* ---- parity hooks: the engine's internal tables --------------------------
  * These expose what the reference keeps in meso_atom / MesoNeighList device
  * arrays (atom_meso.cu:271-387, neigh_build_meso.cu:17-127), in the engine's

@@ -108,6 +108,9 @@ void type_bounds(const std::string &s, int nmax, int &lo, int &hi)
     if (lo < 1 || hi > nmax || lo > hi) fail(MDPD_E_ARG, "numeric type range '%s' out of 1..%d", s.c_str(), nmax);
 }
 
+enum Phase { PH_REBUILD = 0, PH_DENSITY, PH_FORCE, PH_INITIAL, PH_COUNT };
+const char *const PHASE_NAMES[PH_COUNT] = {"rebuild", "density", "force", "initial"};
+
 struct FixDesc {
     std::string id, style;
     int groupbit = 0;
@@ -168,8 +171,17 @@ struct mdpd_system {
     DevBuf<SysDev> dsys;
     int *pinned = nullptr;            // host-visible readback of flags
 
+    // ---- kernel timing with HIP events (phase -> total ms, launches) ----
+    bool prof = false;
+    std::vector<hipEvent_t> ev_free;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open;
+    double prof_ms[PH_COUNT] = {0};
+    long long prof_n[PH_COUNT] = {0};
+
     ~mdpd_system()
     {
+        for (auto &e : ev_open) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
+        for (auto e : ev_free) (void)hipEventDestroy(e);
         if (pinned) (void)hipHostFree(pinned);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -671,6 +683,7 @@ struct mdpd_system {
         hip_check(launch_signatures(P, stream), "signatures");
         force_pass(MODE_SETUP);
         hip_check(hipStreamSynchronize(stream), "setup");
+        collect_events();
         setup_done = true;
         dirty = false;
     }
@@ -679,7 +692,7 @@ struct mdpd_system {
     void rebuild()
     {
         PoolDev P = view();
-        hip_check(launch_rebuild(P, stream), "rebuild");
+        timed(PH_REBUILD, [&]() { hip_check(launch_rebuild(P, stream), "rebuild"); });
         for (int attempt = 0;; ++attempt) {
             int ovf = read_field(offsetof(SysDev, overflow));
             if (!ovf) break;
@@ -699,6 +712,37 @@ struct mdpd_system {
         ++nbuild;
     }
 
+    hipEvent_t take_event()
+    {
+        if (!ev_free.empty()) { hipEvent_t e = ev_free.back(); ev_free.pop_back(); return e; }
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+    template <typename F>
+    void timed(int phase, F launch)
+    {
+        if (!prof) { launch(); return; }
+        hipEvent_t a = take_event(), b = take_event();
+        hip_check(hipEventRecord(a, stream), "hipEventRecord");
+        launch();
+        hip_check(hipEventRecord(b, stream), "hipEventRecord");
+        ev_open.push_back({phase, {a, b}});
+    }
+    // fold finished event pairs into the per-phase totals (stream synchronized)
+    void collect_events()
+    {
+        for (auto &e : ev_open) {
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, e.second.first, e.second.second), "hipEventElapsedTime");
+            prof_ms[e.first] += ms;
+            prof_n[e.first] += 1;
+            ev_free.push_back(e.second.first);
+            ev_free.push_back(e.second.second);
+        }
+        ev_open.clear();
+    }
+
     int read_field(size_t offset)
     {
         hip_check(hipMemcpyAsync(pinned, (char *)dsys.p + offset, sizeof(int), hipMemcpyDeviceToHost, stream),
@@ -710,8 +754,9 @@ struct mdpd_system {
     void force_pass(int mode)
     {
         PoolDev P = view();
-        if (pair_style == STYLE_MDPD && hsys.rho_bit) hip_check(launch_density(P, stream), "density");
-        hip_check(launch_force(P, pair_style, mode, ntypes == 1, stream), "force");
+        if (pair_style == STYLE_MDPD && hsys.rho_bit)
+            timed(PH_DENSITY, [&]() { hip_check(launch_density(P, stream), "density"); });
+        timed(PH_FORCE, [&]() { hip_check(launch_force(P, pair_style, mode, ntypes == 1, stream), "force"); });
     }
 
     // LAMMPS Neighbor::decide with the flag computed on the device
@@ -729,7 +774,7 @@ struct mdpd_system {
         if (!setup_done || dirty) setup();
         if (nsteps == 0) return;
         PoolDev P = view();
-        hip_check(launch_initial(P, stream), "initial");
+        timed(PH_INITIAL, [&]() { hip_check(launch_initial(P, stream), "initial"); });
         long long ago = 0;
         for (long long k = 1; k <= nsteps; ++k) {
             ++ago;
@@ -746,6 +791,7 @@ struct mdpd_system {
             force_pass(k < nsteps ? MODE_FUSED : MODE_FINAL);
         }
         hip_check(hipStreamSynchronize(stream), "run");
+        collect_events();
         ntimestep += nsteps;
     }
 
@@ -1040,6 +1086,26 @@ int mdpd_thermo(mdpd_system *sys, double out[4])
         out[1] = (dof * T + vs) / 3.0 / V;
         out[2] = pe;
         out[3] = 0.5 * ke;
+    });
+}
+
+int mdpd_profile(mdpd_system *sys, int enable)
+{
+    MDPD_GUARD(sys, {
+        sys->prof = enable != 0;
+        for (int k = 0; k < PH_COUNT; ++k) { sys->prof_ms[k] = 0; sys->prof_n[k] = 0; }
+    });
+}
+
+int mdpd_profile_get(mdpd_system *sys, const char *phase, double *total_ms, long long *launches)
+{
+    MDPD_GUARD(sys, {
+        if (!phase || !total_ms || !launches) fail(MDPD_E_ARG, "null argument");
+        int k = 0;
+        while (k < PH_COUNT && strcmp(PHASE_NAMES[k], phase) != 0) ++k;
+        if (k == PH_COUNT) fail(MDPD_E_ARG, "unknown phase '%s'", phase);
+        *total_ms = sys->prof_ms[k];
+        *launches = sys->prof_n[k];
     });
 }
 

@@ -83,6 +83,8 @@ def load():
         "mdpd_gather_atoms": (I, [P, ctypes.c_char_p, I, I, P]),
         "mdpd_scatter_atoms": (I, [P, ctypes.c_char_p, I, I, P]),
         "mdpd_thermo": (I, [P, ctypes.POINTER(D)]),
+        "mdpd_profile": (I, [P, I]),
+        "mdpd_profile_get": (I, [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(LL)]),
         "mdpd_debug_sorted": (I, [P, P, P, P]),
         "mdpd_debug_neighbors": (I, [P, ctypes.POINTER(I), P, P]),
         "mdpd_debug_signatures": (I, [P, P]),
@@ -203,6 +205,16 @@ class System:
         out = (ctypes.c_double * 4)()
         self._check(self._lib.mdpd_thermo(self._h, out))
         return {"temp": out[0], "press": out[1], "pe": out[2], "ke": out[3]}
+
+    # -- kernel timing --
+    def profile(self, enable=True):
+        self._check(self._lib.mdpd_profile(self._h, int(bool(enable))))
+
+    def profile_get(self, phase):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._check(self._lib.mdpd_profile_get(self._h, phase.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     # -- parity hooks --
     def debug_sorted(self):
