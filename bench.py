@@ -63,9 +63,11 @@ DECK = """
 """
 
 
-def gpu_run(box, x, v, steps, warmup, seed=4242):
+def gpu_run(box, x, v, steps, warmup, seed=4242, split=None):
     import mdpd355
     s = mdpd355.System(0)
+    if split:
+        s.tune("split", split)
     s.command(f"atom_style mdpd/atomic/meso {RHO}")
     s.create_box(1, (0, 0, 0), box)
     s.create_atoms(np.ones(len(x), np.int32), x, v)
@@ -118,6 +120,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--workload", default="slab", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--split", type=int, default=None, help="waves per tile in the per-step passes")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
@@ -127,7 +130,7 @@ def main():
     box = w["box"]
     x, v = make_slab(box, w["slab"])
     n = len(x)
-    dt, prof, nbuild, thermo = gpu_run(box, x, v, a.steps, a.warmup)
+    dt, prof, nbuild, thermo = gpu_run(box, x, v, a.steps, a.warmup, split=a.split)
     value = n * a.steps / dt
     f_ms, f_n = prof["force"]
     t_force = f_ms / max(f_n, 1) * 1e-3

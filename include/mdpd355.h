@@ -109,6 +109,14 @@ int mdpd_scatter_atoms(mdpd_system *sys, const char *name, int type, int count, 
  * compute_pressure_meso.cu:115-155 */
 int mdpd_thermo(mdpd_system *sys, double out[4]);
 
+/* ---- tuning ----------------------------------------------------------------
+ * Launch-shape knobs of the per-step passes.  No counterpart in the reference,
+ * whose launch sizes come from occupancy tables (src/USER-MESO/occupancy_meso.cu).
+ * "split": waves sharing one 64-particle tile's neighbor rows in the density
+ * and force passes (1, 2 or 4; default 2).  Takes effect at the next pass;
+ * results differ from another split only by FP32 summation order. */
+int mdpd_tune(mdpd_system *sys, const char *knob, int value);
+
 /* ---- kernel timing ---------------------------------------------------------
  * HIP events recorded around every launch on the handle's stream, folded into
  * per-phase totals at the end of each setup / run.  Phases: "rebuild"

@@ -38,6 +38,7 @@ struct SysDev {
     int npad;                    // n rounded up to 64
     int l1_shift;                // 12
     float rc2_list;              // (float)((rc+skin)^2)
+    float rc2_max;               // largest pair cutoff squared (compaction test)
     float trig2;                 // (skin/2)^2
     float dt, dtf, dtinvsqrt;
     uint32_t seed;
@@ -55,11 +56,11 @@ struct SysDev {
     int sela;                    // buffer holding tag, mask, origin, image (flips at re-sort)
     int stale;                   // neighbor list must be rebuilt before the next force pass
     int stale_pending;           // set during a step by the displacement check
-    int done;                    // last-wave commit counter
     int overflow;                // a neighbor row or cell exceeded its capacity
     int active;                  // 0: slot idle (batched / served engine)
     int steps_left;              // served engine: steps still to run
     long long step;              // ntimestep of the next / current force evaluation
+    unsigned long long done;     // last-workgroup commit: arrivals (low 32) + moved flags (high 32)
     uint32_t seed_next;          // premix_TEA<64>(seed, step + 1)
     int pad_;
 };
@@ -87,7 +88,10 @@ struct PoolDev {
     int *cell_sys;       // system of each cell slot
     int *nlist;          // (cap/64) tiles * maxnbr * 64
     int *ncount;         // per particle
+    int *clist;          // per step: entries of nlist within the pair cutoff, same layout
+    int *ccount;         // per particle
     int *wave_sys;       // system of each 64-particle wave
+    int *moved;          // per 64-particle tile: a particle moved past skin/2 this step
     SysDev *sys;
     float *coef;         // pooled coefficient tables
     float *minv;         // pooled per-type 1/m
@@ -189,11 +193,21 @@ __device__ inline float sinpi_half(float t)
     return s * t;
 }
 
-__device__ inline float log1p_kahan(float y)
+// ln(w / 2^32) for 1 <= w < 2^32 without branches: with the leading-bit
+// exponent e (2^e <= w < 2^(e+1)) pick k in {e, e+1} so that a = w / 2^k lies
+// in [1/sqrt2, sqrt2); f = a - 1 is formed from the exact integer difference
+// w - 2^k, and ln u = (k - 32) ln 2 + log1p(f) with Kahan's identity
+// log1p(f) = ln(1+f) * f / ((1+f) - 1), so u near 1 keeps full precision.
+__device__ inline float ln_u32(uint32_t w)
 {
-    float w = 1.0f + y;
-    if (w == 1.0f) return y;
-    return __logf(w) * (y / (w - 1.0f));
+    const int e = 31 - (int)__builtin_clz(w);
+    const bool up = __builtin_amdgcn_ldexpf((float)w, -e) >= 1.41421356f;
+    const int k = up ? e + 1 : e;
+    const uint32_t pk = k >= 32 ? 0u : (1u << k);
+    const float f = __builtin_amdgcn_ldexpf((float)(int)(w - pk), -k);
+    const float w1 = 1.0f + f;
+    const float lp = __logf(w1) * f * __builtin_amdgcn_rcpf(w1 - 1.0f);
+    return (float)(k - 32) * 0.69314718055994530942f + (w1 == 1.0f ? f : lp);
 }
 
 __device__ inline float pair_gaussian(uint32_t si, uint32_t sj)
@@ -204,19 +218,10 @@ __device__ inline float pair_gaussian(uint32_t si, uint32_t sj)
     // x = (v0 & 0x7FFFFFFF) 2^-31 ; t = x - 1/2 exactly from the integer
     const int ti = (int)(v0 & 0x7FFFFFFFu) - (1 << 30);
     const float t = (float)ti * 4.656612873077392578125e-10f;
-    float f = -sinpi_half(t);
-    f = (v0 & 0x80000000u) ? f : -f;
-    const uint32_t w = v1 > 1u ? v1 : 1u;
-    float lnu;
-    if (w >= 0x80000000u) {
-        const float y = -(float)(0u - w) * 2.3283064365386962890625e-10f;
-        lnu = log1p_kahan(y);
-    } else {
-        lnu = (__log2f((float)w) - 32.0f) * 0.69314718055994530942f;
-    }
-    const float r = __fsqrt_rn(-2.0f * lnu);
-    float g = r * f;
-    return fminf(fmaxf(g, -4.0f), 4.0f);
+    const float s = sinpi_half(t);                 // cos(pi x) = -sin(pi t)
+    const float f = (v0 & 0x80000000u) ? -s : s;   // times +1 / -1 by the top bit
+    const float r = __builtin_amdgcn_sqrtf(-2.0f * ln_u32(v1 > 1u ? v1 : 1u));
+    return fminf(fmaxf(r * f, -4.0f), 4.0f);
 }
 
 // image code -> shift multipliers

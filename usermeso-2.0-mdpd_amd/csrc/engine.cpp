@@ -166,13 +166,14 @@ struct mdpd_system {
     DevBuf<int> tag[2], mask[2], origin[2], image[2];
     DevBuf<float> rho, vir, coef, minv, mmass;
     DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys;
-    DevBuf<int> nlist, ncount, wave_sys;
+    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved;
     DevBuf<unsigned long long> key;
     DevBuf<SysDev> dsys;
     int *pinned = nullptr;            // host-visible readback of flags
 
     // ---- kernel timing with HIP events (phase -> total ms, launches) ----
     bool prof = false;
+    int split = 2;                    // waves per 64-particle tile in the per-step passes
     std::vector<hipEvent_t> ev_free;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open;
     double prof_ms[PH_COUNT] = {0};
@@ -499,6 +500,7 @@ struct mdpd_system {
         P.cell_count = cell_count.p; P.cell_start = cell_start.p; P.cell_xyz = cell_xyz.p;
         P.cell_rank = cell_rank.p; P.cell_sys = cell_sys.p;
         P.nlist = nlist.p; P.ncount = ncount.p; P.wave_sys = wave_sys.p;
+        P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
         P.cap = cap; P.ncellslots = nslots; P.nsys = 1; P.maxnbr = maxnbr;
         return P;
@@ -553,6 +555,7 @@ struct mdpd_system {
         d.n = n;
         d.npad = (n + 255) / 256 * 256;
         d.rc2_list = (float)(cutneigh * cutneigh);
+        d.rc2_max = (float)(rcut * rcut);
         d.trig2 = (float)(0.25 * skin * skin);
         d.dt = (float)dt;
         d.dtf = (float)(0.5 * dt);
@@ -621,9 +624,10 @@ struct mdpd_system {
         xhold.alloc(cap); frc.alloc(cap); vend.alloc(cap);
         rho.alloc(cap); vir.alloc((size_t)6 * cap);
         pcell.alloc(cap); pslot.alloc(cap); psub.alloc(cap); tmp.alloc(cap); key.alloc(2 * (size_t)cap);
-        ncount.alloc(cap);
+        ncount.alloc(cap); ccount.alloc((size_t)4 * cap);
         nlist.alloc((size_t)cap * maxnbr);
-        wave_sys.alloc(cap / WAVE);
+        clist.alloc((size_t)cap * maxnbr);
+        wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE);
         cell_count.alloc(nslots); cell_start.alloc(nslots); cell_xyz.alloc(nslots);
         cell_rank.alloc(nslots); cell_sys.alloc(nslots);
         coef.alloc(tab.size()); minv.alloc(ntypes); mmass.alloc(ntypes); dsys.alloc(1);
@@ -700,6 +704,7 @@ struct mdpd_system {
             // grow rows and rebuild the rows only (sorted data is in place)
             maxnbr = (int)std::ceil(maxnbr * 1.5 / 16.0) * 16;
             nlist.alloc((size_t)cap * maxnbr);
+            clist.alloc((size_t)cap * maxnbr);
             int zero = 0;
             hip_check(hipMemcpyAsync((char *)dsys.p + offsetof(SysDev, overflow), &zero, sizeof(int),
                                      hipMemcpyHostToDevice, stream), "clear overflow");
@@ -754,9 +759,10 @@ struct mdpd_system {
     void force_pass(int mode)
     {
         PoolDev P = view();
-        if (pair_style == STYLE_MDPD && hsys.rho_bit)
-            timed(PH_DENSITY, [&]() { hip_check(launch_density(P, stream), "density"); });
-        timed(PH_FORCE, [&]() { hip_check(launch_force(P, pair_style, mode, ntypes == 1, stream), "force"); });
+        const bool with_rho = pair_style == STYLE_MDPD && hsys.rho_bit;
+        timed(PH_DENSITY, [&]() { hip_check(launch_first(P, with_rho, split, stream), "first pass"); });
+        timed(PH_FORCE, [&]() { hip_check(launch_force(P, pair_style, mode, ntypes == 1, split, stream), "force"); });
+        if (mode != MODE_SETUP) hip_check(launch_commit_step(P, mode != MODE_FINAL, stream), "commit");
     }
 
     // LAMMPS Neighbor::decide with the flag computed on the device
@@ -775,6 +781,7 @@ struct mdpd_system {
         if (nsteps == 0) return;
         PoolDev P = view();
         timed(PH_INITIAL, [&]() { hip_check(launch_initial(P, stream), "initial"); });
+        hip_check(launch_commit_step(P, true, stream), "commit");
         long long ago = 0;
         for (long long k = 1; k <= nsteps; ++k) {
             ++ago;
@@ -1086,6 +1093,19 @@ int mdpd_thermo(mdpd_system *sys, double out[4])
         out[1] = (dof * T + vs) / 3.0 / V;
         out[2] = pe;
         out[3] = 0.5 * ke;
+    });
+}
+
+int mdpd_tune(mdpd_system *sys, const char *knob, int value)
+{
+    MDPD_GUARD(sys, {
+        if (!knob) fail(MDPD_E_ARG, "null knob");
+        if (!strcmp(knob, "split")) {
+            if (value != 1 && value != 2 && value != 4) fail(MDPD_E_ARG, "split must be 1, 2 or 4");
+            sys->split = value;
+        } else {
+            fail(MDPD_E_ARG, "unknown tuning knob '%s'", knob);
+        }
     });
 }
 

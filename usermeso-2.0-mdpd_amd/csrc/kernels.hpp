@@ -19,9 +19,12 @@ hipError_t launch_commit_rebuild(const PoolDev &P, hipStream_t st);
 hipError_t launch_signatures(const PoolDev &P, hipStream_t st);
 
 // per step
-hipError_t launch_density(const PoolDev &P, hipStream_t st);
+// first pass of a step: density (mDPD with the density fix) and compact rows
+hipError_t launch_first(const PoolDev &P, bool with_rho, int split, hipStream_t st);
 hipError_t launch_initial(const PoolDev &P, hipStream_t st);
-hipError_t launch_force(const PoolDev &P, int style, int mode, bool ev, hipStream_t st);
+// after launch_initial / a fused force pass (advance) or a final pass (!advance)
+hipError_t launch_commit_step(const PoolDev &P, bool advance, hipStream_t st);
+hipError_t launch_force(const PoolDev &P, int style, int mode, bool uni, int split, hipStream_t st);
 
 // stateless helpers for parity tests
 hipError_t launch_pair_gaussian(const uint32_t *a, const uint32_t *b, float *out, int n, hipStream_t st);

@@ -83,30 +83,49 @@ __global__ void __launch_bounds__(1024) k_scan(PoolDev P)
     const int s = blockIdx.x;
     SysDev *d = P.sys + s;
     if (!d->active || !d->stale) return;
-    __shared__ int part[1024];
+    // chunks of 8192 counts: coalesced load into LDS, 8 consecutive counts per
+    // thread, wave scan + scan of the 16 wave totals, coalesced store
+    constexpr int PER = 8, CH = 1024 * PER;
+    __shared__ int buf[CH];
+    __shared__ int wex[16];
+    __shared__ int carry;
     const int nc = d->ncell, base = d->cell_base;
-    const int per = (nc + 1023) / 1024;
-    const int lo = threadIdx.x * per;
-    const int hi = min(lo + per, nc);
-    int sum = 0;
-    for (int k = lo; k < hi; ++k) sum += P.cell_count[base + k];
-    part[threadIdx.x] = sum;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) carry = d->part_base;
     __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partial sums
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    for (int c0 = 0; c0 < nc; c0 += CH) {
+        for (int k = t; k < CH; k += 1024) {
+            const int c = c0 + k;
+            int v = 0;
+            if (c < nc) { v = P.cell_count[base + c]; P.cell_count[base + c] = 0; }
+            buf[k] = v;
+        }
         __syncthreads();
-        part[threadIdx.x] += v;
+        int v[PER], sum = 0;
+        for (int k = 0; k < PER; ++k) { v[k] = buf[t * PER + k]; sum += v[k]; }
+        int incl = sum;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        if (lane == 63) wex[w] = incl;
+        __syncthreads();
+        if (t == 0) {
+            int acc = 0;
+            for (int k = 0; k < 16; ++k) { const int y = wex[k]; wex[k] = acc; acc += y; }
+        }
+        __syncthreads();
+        int run = carry + wex[w] + incl - sum;
+        for (int k = 0; k < PER; ++k) { buf[t * PER + k] = run; run += v[k]; }
+        __syncthreads();
+        for (int k = t; k < CH; k += 1024) {
+            const int c = c0 + k;
+            if (c < nc) P.cell_start[base + c] = buf[k];
+        }
+        if (t == 1023) carry = run;
         __syncthreads();
     }
-    int run = d->part_base + part[threadIdx.x] - sum;
-    for (int k = lo; k < hi; ++k) {
-        const int cnt = P.cell_count[base + k];
-        P.cell_start[base + k] = run;
-        P.cell_count[base + k] = 0;
-        run += cnt;
-    }
-    if (threadIdx.x == 0) P.cell_start[base + nc] = d->part_base + d->n;
+    if (t == 0) P.cell_start[base + nc] = d->part_base + d->n;
 }
 
 // --------------------------------------------------------------------------
@@ -188,16 +207,39 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 // (neigh_build_meso.cu:86-93).  Row layout: tile (i/64), entry p at
 // tile*maxnbr*64 + p*64 + i%64.
 // --------------------------------------------------------------------------
-constexpr int K5_CENTRES = 128;
+constexpr int K5_CH = 16;          // centres per LDS chunk (per wave)
+constexpr int K5_WAVES = 4;        // independent waves (cells) per workgroup
 
-__global__ void __launch_bounds__(256) k_nlist(PoolDev P)
+// per-wave LDS words: centres (4 each), counts, stencil start/prefix/code,
+// stencil shifts (3 each), rows (odd stride: conflict-free column reads)
+static inline __host__ __device__ int nlist_wave_words(int maxnbr)
 {
-    __shared__ float4 s_ci[4][K5_CENTRES];
-    __shared__ int s_cnt[4][K5_CENTRES];
-    __shared__ int s_start[4][32], s_pre[4][32], s_code[4][32];
-    __shared__ float s_sh[4][32][3];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int slot = blockIdx.x * 4 + wv;
+    return K5_CH * 4 + K5_CH + 32 * 3 + 32 * 3 + K5_CH * (maxnbr + 1);
+}
+
+static inline size_t nlist_lds_bytes(int maxnbr) { return (size_t)K5_WAVES * nlist_wave_words(maxnbr) * 4; }
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ void __launch_bounds__(WAVE * K5_WAVES) k_nlist(PoolDev P)
+{
+    extern __shared__ int smem[];
+    const int maxnbr = P.maxnbr;
+    const int stride = maxnbr + 1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int *base_w = smem + wv * nlist_wave_words(maxnbr);
+    float4 *s_ci = (float4 *)base_w;                   // K5_CH
+    int *s_cnt = base_w + K5_CH * 4;                   // K5_CH
+    int *s_start = s_cnt + K5_CH;                      // 32
+    int *s_pre = s_start + 32;                         // 32
+    int *s_code = s_pre + 32;                          // 32
+    float *s_sh = (float *)(s_code + 32);              // 32 * 3
+    int *s_rows = (int *)(s_sh + 96);                  // K5_CH * stride
+    const int slot = blockIdx.x * K5_WAVES + wv;
     if (slot >= P.ncellslots) return;
     const int s = P.cell_sys[slot];
     if (s < 0) return;
@@ -210,7 +252,6 @@ __global__ void __launch_bounds__(256) k_nlist(PoolDev P)
     if (m <= 0) return;
     const int nxt = d->sel ^ 1;      // sorted data written by k_sortgather
     const float4 *pos = P.pos[nxt];
-    const int maxnbr = P.maxnbr;
 
     // stencil: lane k < 27 describes neighbor cell k
     const int xyz = P.cell_xyz[slot];
@@ -226,33 +267,30 @@ __global__ void __launch_bounds__(256) k_nlist(PoolDev P)
         const int ns = d->cell_base + P.cell_rank[d->cell_base + lin];
         const int st = P.cell_start[ns];
         len = P.cell_start[ns + 1] - st;
-        s_start[wv][lane] = st;
-        s_code[wv][lane] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-        s_sh[wv][lane][0] = (float)sx * d->L[0];
-        s_sh[wv][lane][1] = (float)sy * d->L[1];
-        s_sh[wv][lane][2] = (float)sz * d->L[2];
+        s_start[lane] = st;
+        s_code[lane] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+        s_sh[lane * 3 + 0] = (float)sx * d->L[0];
+        s_sh[lane * 3 + 1] = (float)sy * d->L[1];
+        s_sh[lane * 3 + 2] = (float)sz * d->L[2];
     }
-    // exclusive prefix of the 27 lengths
     int incl = len;
     for (int off = 1; off < 32; off <<= 1) {
         const int v = __shfl_up(incl, off, 64);
         if (lane >= off) incl += v;
     }
-    if (lane < 27) s_pre[wv][lane] = incl - len;
+    if (lane < 27) s_pre[lane] = incl - len;
     const int total = __shfl(incl, 26, 64);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
 
     const float rc2 = d->rc2_list;
     int any_over = 0;
-    for (int c0 = 0; c0 < m; c0 += K5_CENTRES) {
-        const int mc = min(K5_CENTRES, m - c0);
-        for (int c = lane; c < mc; c += WAVE) {
-            s_ci[wv][c] = pos[start + c0 + c];
-            s_cnt[wv][c] = 0;
+    for (int c0 = 0; c0 < m; c0 += K5_CH) {
+        const int mc = min(K5_CH, m - c0);
+        if (lane < mc) {
+            s_ci[lane] = pos[start + c0 + lane];
+            s_cnt[lane] = 0;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+        wave_sync();
         for (int b0 = 0; b0 < total; b0 += WAVE) {
             const int mi = b0 + lane;
             const bool valid = mi < total;
@@ -262,48 +300,57 @@ __global__ void __launch_bounds__(256) k_nlist(PoolDev P)
                 int lo = 0, hi = 26;
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
-                    if (s_pre[wv][mid] <= mi) lo = mid; else hi = mid - 1;
+                    if (s_pre[mid] <= mi) lo = mid; else hi = mid - 1;
                 }
                 k = lo;
             }
             int j = 0, code = IMG_ZERO;
             float xj = 0.f, yj = 0.f, zj = 0.f;
             if (valid) {
-                j = s_start[wv][k] + (mi - s_pre[wv][k]);
-                code = s_code[wv][k];
+                j = s_start[k] + (mi - s_pre[k]);
+                code = s_code[k];
                 const float4 pj = pos[j];
-                xj = pj.x + s_sh[wv][k][0];
-                yj = pj.y + s_sh[wv][k][1];
-                zj = pj.z + s_sh[wv][k][2];
+                xj = pj.x + s_sh[k * 3 + 0];
+                yj = pj.y + s_sh[k * 3 + 1];
+                zj = pj.z + s_sh[k * 3 + 2];
             }
-            const uint32_t entry = (uint32_t)j | ((uint32_t)code << IMG_SHIFT);
+            const int entry = (int)((uint32_t)j | ((uint32_t)code << IMG_SHIFT));
             for (int c = 0; c < mc; ++c) {
-                const float4 ci = s_ci[wv][c];
+                const float4 ci = s_ci[c];
                 const float dx = ci.x - xj, dy = ci.y - yj, dz = ci.z - zj;
                 const float r2 = (dx * dx + dy * dy) + dz * dz;
                 const int ic = start + c0 + c;
                 const bool hit = valid && r2 <= rc2 && !(j == ic && code == IMG_ZERO);
                 const uint64_t bal = __ballot(hit);
                 if (bal) {
-                    const int n0 = s_cnt[wv][c];
+                    const int n0 = s_cnt[c];
                     const int p = n0 + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                               __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                    if (hit && p < maxnbr)
-                        P.nlist[(size_t)(ic >> 6) * maxnbr * WAVE + (size_t)p * WAVE + (ic & 63)] = (int)entry;
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane == 0) s_cnt[wv][c] = n0 + __popcll(bal);
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
+                    if (hit && p < maxnbr) s_rows[c * stride + p] = entry;
+                    wave_sync();
+                    if (lane == 0) s_cnt[c] = n0 + __popcll(bal);
+                    wave_sync();
                 }
             }
         }
-        for (int c = lane; c < mc; c += WAVE) {
-            const int n = s_cnt[wv][c];
-            P.ncount[start + c0 + c] = n;
-            any_over |= n > maxnbr;
+        // counts, then rows: lane = centre (lane & 15) and row offset (lane >> 4)
+        int cntc = 0;
+        if (lane < mc) {
+            cntc = s_cnt[lane];
+            P.ncount[start + c0 + lane] = cntc;
+            any_over |= cntc > maxnbr;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+        int rmax = min(cntc, maxnbr);
+        for (int off = 32; off > 0; off >>= 1) rmax = max(rmax, __shfl_xor(rmax, off, 64));
+        const int c = lane & (K5_CH - 1);
+        const int myc = c < mc ? min(s_cnt[c], maxnbr) : 0;
+        const int ic = start + c0 + c;
+        const size_t base = (size_t)(ic >> 6) * maxnbr * WAVE + (ic & 63);
+        for (int p0 = 0; p0 < rmax; p0 += WAVE / K5_CH) {
+            const int p = p0 + lane / K5_CH;
+            if (p < myc) P.nlist[base + (size_t)p * WAVE] = s_rows[c * stride + p];
+        }
+        wave_sync();
     }
     if (__ballot(any_over) && lane == 0) atomicOr(&d->overflow, 1);
 }
@@ -354,13 +401,13 @@ hipError_t launch_rebuild(const PoolDev &P, hipStream_t st)
     hipLaunchKernelGGL(k_scan, dim3(P.nsys), dim3(1024), 0, st, P);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(P.cap, 256)), dim3(256), 0, st, P);
     hipLaunchKernelGGL(k_sortgather, dim3(nblk(P.ncellslots, 4)), dim3(256), 0, st, P);
-    hipLaunchKernelGGL(k_nlist, dim3(nblk(P.ncellslots, 4)), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(k_nlist, dim3(nblk(P.ncellslots, K5_WAVES)), dim3(WAVE * K5_WAVES), nlist_lds_bytes(P.maxnbr), st, P);
     return hipGetLastError();
 }
 
 hipError_t launch_nlist_only(const PoolDev &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_nlist, dim3(nblk(P.ncellslots, 4)), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(k_nlist, dim3(nblk(P.ncellslots, K5_WAVES)), dim3(WAVE * K5_WAVES), nlist_lds_bytes(P.maxnbr), st, P);
     return hipGetLastError();
 }
 

@@ -9,9 +9,15 @@
 //   MesoFixAddForce::post_force                             fix_addforce_meso.cu:72-108
 //   dp2sp_merged signatures                                 atom_vec_mdpd_atomic_meso.cu:140-206
 //
-// Launch shape: one thread per particle, 256-thread workgroups; each system
-// occupies whole workgroups (systems are padded to 256 slots), and workgroup
-// ids are remapped so each of the 8 XCDs sweeps one contiguous particle range.
+// Each step the first pass (density for mDPD, a filter pass for DPD) walks the
+// full neighbor rows (cutoff + skin) and writes, per particle, the compact row
+// of entries inside the pair cutoff; the force pass then walks only those.
+// Compact rows are staged per lane in LDS and stored as whole coalesced rows.
+//
+// Launch shape: the first and force passes run one workgroup per 64-particle
+// tile with SPLIT waves sharing the tile's rows; systems are padded to 256
+// slots; workgroup ids are remapped so each of the 8 XCDs sweeps one
+// contiguous particle range.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -30,29 +36,18 @@ __device__ inline int xcd_logical(int b, int nlogical)
 
 static inline int xcd_grid(int nlogical) { return ((nlogical + NXCD - 1) / NXCD) * NXCD; }
 
-// Last-workgroup commit.  Every workgroup of a system calls this once after
-// finishing all of its reads and writes; the last one to arrive applies
-// `commit` with no other workgroup of that system still running.
-template <typename F>
-__device__ inline void block_commit(SysDev *d, int pending_or, F commit)
+// Per-tile "a particle moved past skin/2" flags: each workgroup ORs its lanes
+// in LDS and stores one word per 64-particle tile it covers (plain stores, no
+// atomics: same-address device-scope atomics serialize across the XCDs and
+// cost ~50 ns each).  k_commit_step reduces them per system after the pass.
+__device__ inline void tile_flags(int *moved, int tile0, int ntile, int flag)
 {
-    __shared__ int s_or;
-    if (threadIdx.x == 0) s_or = 0;
+    __shared__ int s_or[4];
+    if (threadIdx.x < 4) s_or[threadIdx.x] = 0;
     __syncthreads();
-    if (pending_or) atomicOr(&s_or, 1);
+    if (flag) s_or[(threadIdx.x >> 6) % ntile] = 1;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        if (s_or) atomicOr(&d->stale_pending, 1);
-        __threadfence();
-        const int nblk = d->npad / BLK;
-        const int old = atomicAdd(&d->done, 1);
-        if (old == nblk - 1) {
-            __threadfence();
-            commit();
-            d->done = 0;
-            __threadfence();
-        }
-    }
+    if (threadIdx.x < ntile) moved[tile0 + threadIdx.x] = s_or[threadIdx.x];
 }
 
 __device__ inline void apply_image(uint32_t code, const SysDev *d, float &x, float &y, float &z)
@@ -64,61 +59,99 @@ __device__ inline void apply_image(uint32_t code, const SysDev *d, float &x, flo
     z += (float)sz * d->L[2];
 }
 
-// --------------------------------------------------------------------------
-// density pass: rho_i = m_i * sum_j w(r_ij), 1e-20 <= r^2 < rd^2, no self
-// --------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLK) k_density(PoolDev P, int nlogical)
+__device__ inline float pair_rsq(const float4 &pi, float4 q, uint32_t code, const SysDev *d)
 {
-    const int lb = xcd_logical(blockIdx.x, nlogical);
-    if (lb >= nlogical) return;
-    const int i = lb * BLK + threadIdx.x;
-    const int s = P.wave_sys[i >> 6];
+    float xj = q.x, yj = q.y, zj = q.z;
+    if (code != IMG_ZERO) apply_image(code, d, xj, yj, zj);
+    const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// --------------------------------------------------------------------------
+// first pass: density rho_i = m_i * sum_j w(r_ij) over 1e-20 <= r^2 < rd^2
+// (no self term; fix_aboundary_rho_meso.cu:131-164), and the compact rows of
+// entries with r^2 < rc^2 for the force pass.  WITH_RHO = false is the DPD
+// filter pass (no density).
+//
+// One workgroup per 64-particle tile; wave w of SPLIT walks rows p = w mod
+// SPLIT of the tile and writes its compact entries to compact rows
+// t * SPLIT + w, so the force pass can give the same rows to the same wave.
+// Partial densities are summed in LDS in wave order (deterministic).
+// --------------------------------------------------------------------------
+template <bool WITH_RHO, int SPLIT>
+__global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
+{
+    constexpr int STAGE = 32 / SPLIT;
+    __shared__ int s_c[SPLIT][STAGE * WAVE];
+    __shared__ float s_sum[SPLIT][WAVE];
+    const int tile = xcd_logical(blockIdx.x, ntiles);
+    if (tile >= ntiles) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = tile * WAVE + lane;
+    const int s = P.wave_sys[tile];
     if (s < 0) return;
     const SysDev *d = P.sys + s;
-    if (!d->active || i >= d->part_base + d->n) return;
-    const int cur = d->sel;
-    const float4 *pos = P.pos[cur];
-    const float4 pi = pos[i];
-    const int mk = P.mask[d->sela][i];
-    if (!(mk & d->rho_bit)) { P.rho[i] = 0.f; return; }
+    if (!d->active) return;
+    const bool valid = i < d->part_base + d->n;
+    const float4 *pos = P.pos[d->sel];
     const int maxnbr = P.maxnbr;
-    const int cnt = min(P.ncount[i], maxnbr);
-    const int *row = P.nlist + (size_t)(i >> 6) * maxnbr * WAVE + (i & 63);
+    const size_t tbase = (size_t)tile * maxnbr * WAVE + lane;
+    const int *row = P.nlist + tbase;
+    int *crow = P.clist + tbase;
+    int cnt = 0;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid) {
+        pi = pos[i];
+        cnt = min(P.ncount[i], maxnbr);
+    }
     const float h = d->rd, h2 = h * h;
+    const float rc2 = d->rc2_max;
+    const bool lucy = d->lucy;
     float sum = 0.f;
-    if (d->lucy) {
-        for (int p = 0; p < cnt; ++p) {
-            const uint32_t e = (uint32_t)row[(size_t)p * WAVE];
-            const uint32_t j = e & IDX_MASK, code = e >> IMG_SHIFT;
-            const float4 q = pos[j];
-            float xj = q.x, yj = q.y, zj = q.z;
-            if (code != IMG_ZERO) apply_image(code, d, xj, yj, zj);
-            const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-            const float rsq = dx * dx + dy * dy + dz * dz;
-            if (rsq < h2 && rsq >= 1e-20f) {
-                const float r = __fsqrt_rn(rsq);
-                const float hm = h - r;
-                sum += (h + 3.f * r) * hm * hm * hm;
-            }
+    int nc = 0;
+    auto visit = [&](uint32_t e, const float4 &q) {
+        const float rsq = pair_rsq(pi, q, e >> IMG_SHIFT, d);
+        if (WITH_RHO && rsq < h2 && rsq >= 1e-20f) {
+            const float r = __builtin_amdgcn_sqrtf(rsq);
+            const float hm = h - r;
+            sum += lucy ? (h + 3.f * r) * hm * hm * hm : hm * hm;
         }
-    } else {
-        for (int p = 0; p < cnt; ++p) {
-            const uint32_t e = (uint32_t)row[(size_t)p * WAVE];
-            const uint32_t j = e & IDX_MASK, code = e >> IMG_SHIFT;
-            const float4 q = pos[j];
-            float xj = q.x, yj = q.y, zj = q.z;
-            if (code != IMG_ZERO) apply_image(code, d, xj, yj, zj);
-            const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-            const float rsq = dx * dx + dy * dy + dz * dz;
-            if (rsq < h2 && rsq >= 1e-20f) {
-                const float r = __fsqrt_rn(rsq);
-                const float hm = h - r;
-                sum += hm * hm;
-            }
+        if (rsq < rc2) {
+            if (nc < STAGE) s_c[w][nc * WAVE + lane] = (int)e;
+            else crow[(size_t)(nc * SPLIT + w) * WAVE] = (int)e;
+            ++nc;
+        }
+    };
+    int p = w;
+    for (; p + SPLIT < cnt; p += 2 * SPLIT) {
+        const uint32_t e0 = (uint32_t)row[(size_t)p * WAVE];
+        const uint32_t e1 = (uint32_t)row[(size_t)(p + SPLIT) * WAVE];
+        const float4 q0 = pos[e0 & IDX_MASK];
+        const float4 q1 = pos[e1 & IDX_MASK];
+        visit(e0, q0);
+        visit(e1, q1);
+    }
+    if (p < cnt) {
+        const uint32_t e0 = (uint32_t)row[(size_t)p * WAVE];
+        visit(e0, pos[e0 & IDX_MASK]);
+    }
+    // coalesced store of the staged rows
+    int ncmax = min(nc, STAGE);
+    for (int off = 32; off > 0; off >>= 1) ncmax = max(ncmax, __shfl_xor(ncmax, off, 64));
+    for (int t = 0; t < ncmax; ++t)
+        if (t < nc) crow[(size_t)(t * SPLIT + w) * WAVE] = s_c[w][t * WAVE + lane];
+    P.ccount[((size_t)tile * SPLIT + w) * WAVE + lane] = nc;
+    if (WITH_RHO) {
+        s_sum[w][lane] = sum;
+        __syncthreads();
+        if (w == 0 && valid) {
+            float tot = s_sum[0][lane];
+            for (int k = 1; k < SPLIT; ++k) tot += s_sum[k][lane];
+            const int mk = P.mask[d->sela][i];
+            const int ti = __float_as_int(pi.w);
+            P.rho[i] = (mk & d->rho_bit) ? P.mass[d->mass_base + ti] * d->rho_factor * tot : 0.f;
         }
     }
-    const int ti = __float_as_int(pi.w);
-    P.rho[i] = P.mass[d->mass_base + ti] * d->rho_factor * sum;
 }
 
 // --------------------------------------------------------------------------
@@ -129,13 +162,12 @@ __device__ inline uint32_t next_signature(const SysDev *d, int tag, float vx, fl
     return d->style == STYLE_MDPD ? sig_mdpd(tag, d->seed_next) : sig_dpd(tag, d->seed_next, vx, vy, vz);
 }
 
-__device__ inline void commit_advance(SysDev *d)
+__device__ inline void commit_advance(SysDev *d, bool moved)
 {
     d->sel ^= 1;
     d->step += 1;
     d->seed_next = premix<64>(d->seed, (uint32_t)(d->step + 1));
-    d->stale = d->stale_pending;
-    d->stale_pending = 0;
+    d->stale = moved ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(BLK) k_initial(PoolDev P, int nlogical)
@@ -166,103 +198,150 @@ __global__ void __launch_bounds__(BLK) k_initial(PoolDev P, int nlogical)
         const float ex = p.x - h.x, ey = p.y - h.y, ez = p.z - h.z;
         moved = ex * ex + ey * ey + ez * ez > d->trig2;
     }
-    block_commit(d, moved, [&]() { commit_advance(d); });
+    tile_flags(P.moved, lb * (BLK / WAVE), BLK / WAVE, moved);
 }
 
 // --------------------------------------------------------------------------
-// pair force (+ fused velocity-Verlet)
+// pair force (+ fused velocity-Verlet) over the compact rows.  One workgroup
+// per 64-particle tile; wave w of SPLIT takes compact rows t * SPLIT + w (the
+// ones the first pass wrote for it); partial sums meet in LDS and wave 0 adds
+// them in wave order, then runs the integration epilogue.
 // --------------------------------------------------------------------------
-struct Pair6 {
-    float rc2, rcinv, rdinv, a, b, gamma, sigma, expw;
+struct PairC {
+    float rc, rc2, rcinv, rd, rdinv, a, b, gamma, sigma, expw;
 };
 
-__device__ inline Pair6 load_coef(const float *tab, int idx)
+__device__ inline PairC load_coef(const float *tab, int idx)
 {
     const float *c = tab + idx * NCOEF;
-    Pair6 q;
-    q.rc2 = c[C_RC2]; q.rcinv = c[C_RCINV]; q.rdinv = c[C_RDINV];
+    PairC q;
+    q.rc = c[C_RC]; q.rc2 = c[C_RC2]; q.rcinv = c[C_RCINV]; q.rd = c[C_RD]; q.rdinv = c[C_RDINV];
     q.a = c[C_A]; q.b = c[C_B]; q.gamma = c[C_GAMMA]; q.sigma = c[C_SIGMA]; q.expw = c[C_EXPW];
     return q;
 }
 
-template <int STYLE, int MODE, bool UNI>
-__global__ void __launch_bounds__(BLK) k_force(PoolDev P, int nlogical)
+template <bool EV>
+struct Acc {
+    static constexpr int N = EV ? 10 : 3;
+    float v[N];
+    __device__ Acc() { for (int k = 0; k < N; ++k) v[k] = 0.f; }
+};
+
+template <int STYLE, bool EV>
+__device__ inline void pair_term(Acc<EV> &a, const float4 &pi, const float4 &vi, float rho_i, const float4 &q,
+                                 const float4 &vj, float rho_j, uint32_t code, const PairC &c, float dtis,
+                                 const SysDev *d)
+{
+    float xj = q.x, yj = q.y, zj = q.z;
+    if (code != IMG_ZERO) apply_image(code, d, xj, yj, zj);
+    const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+    const float rsq = dx * dx + dy * dy + dz * dz;
+    const bool in = rsq < c.rc2 && rsq >= 1e-20f;
+    const float rinv = __builtin_amdgcn_rsqf(rsq);
+    const float r = rsq * rinv;
+    const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+    const float dot = dx * dvx + dy * dvy + dz * dvz;
+    const float wc = 1.f - r * c.rcinv;
+    const float xi = pair_gaussian(__float_as_uint(vi.w), __float_as_uint(vj.w));
+    float fpair, en = 0.f;
+    if (STYLE == STYLE_MDPD) {
+        // pair_mdpd_meso.cu:156-191
+        const float wcr = fmaxf(1.f - r * c.rdinv, 0.f);
+        const float rr = rho_i + rho_j;
+        fpair = ((c.a * wc + c.b * rr * wcr) - c.gamma * wc * wc * dot * rinv + c.sigma * wc * xi * dtis) * rinv;
+        if (EV) en = 0.5f * c.a * c.rc * wc * wc + 0.5f * c.b * c.rd * rr * wcr * wcr;
+    } else {
+        // pair_dpd_meso.cu:145-175
+        const float wr = c.expw == 1.f ? wc : powf(wc, c.expw);
+        fpair = (c.a * wc - c.gamma * wr * wr * dot * rinv + c.sigma * wr * xi * dtis) * rinv;
+        if (EV) en = 0.5f * c.a * c.rc * wc * wc;
+    }
+    fpair = in ? fpair : 0.f;
+    a.v[0] += dx * fpair; a.v[1] += dy * fpair; a.v[2] += dz * fpair;
+    if (EV) {
+        a.v[3] += in ? en : 0.f;
+        a.v[4] += dx * dx * fpair; a.v[5] += dy * dy * fpair; a.v[6] += dz * dz * fpair;
+        a.v[7] += dx * dy * fpair; a.v[8] += dx * dz * fpair; a.v[9] += dy * dz * fpair;
+    }
+}
+
+template <int STYLE, int MODE, bool UNI, int SPLIT>
+__global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int ntiles)
 {
     constexpr bool EV = MODE != MODE_FUSED;
-    const int lb = xcd_logical(blockIdx.x, nlogical);
-    if (lb >= nlogical) return;
-    const int i = lb * BLK + threadIdx.x;
-    const int s = P.wave_sys[(lb * BLK) >> 6];
+    constexpr int NA = Acc<EV>::N;
+    __shared__ float s_acc[SPLIT > 1 ? SPLIT - 1 : 1][NA][WAVE];
+    const int tile = xcd_logical(blockIdx.x, ntiles);
+    if (tile >= ntiles) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = tile * WAVE + lane;
+    const int s = P.wave_sys[tile];
     if (s < 0) return;
     SysDev *d = P.sys + s;
     if (!d->active) return;
     if (UNI != (d->ntypes == 1)) return;     // host launches the matching variant
+    const bool valid = i < d->part_base + d->n;
+    const int cur = d->sel;
+    const float4 *pos = P.pos[cur];
+    const float4 *vel = P.vel[cur];
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
+    float rho_i = 0.f;
+    int ti = 0, cnt = 0;
+    if (valid) {
+        pi = pos[i];
+        vi = vel[i];
+        ti = __float_as_int(pi.w);
+        if (STYLE == STYLE_MDPD) rho_i = P.rho[i];
+        cnt = P.ccount[((size_t)tile * SPLIT + w) * WAVE + lane];
+    }
+    const float dtis = d->dtinvsqrt;
+    const float *coef = P.coef + d->coef_base;
+    const int nt = d->ntypes;
+    PairC cu;
+    if (UNI) cu = load_coef(coef, 0);
+    const int maxnbr = P.maxnbr;
+    const int *crow = P.clist + (size_t)tile * maxnbr * WAVE + lane;
+    Acc<EV> a;
+    int t = 0;
+    for (; t + 2 <= cnt; t += 2) {
+        const uint32_t e0 = (uint32_t)crow[(size_t)(t * SPLIT + w) * WAVE];
+        const uint32_t e1 = (uint32_t)crow[(size_t)((t + 1) * SPLIT + w) * WAVE];
+        const uint32_t j0 = e0 & IDX_MASK, j1 = e1 & IDX_MASK;
+        const float4 q0 = pos[j0], q1 = pos[j1];
+        const float4 w0 = vel[j0], w1 = vel[j1];
+        const float r0 = STYLE == STYLE_MDPD ? P.rho[j0] : 0.f;
+        const float r1 = STYLE == STYLE_MDPD ? P.rho[j1] : 0.f;
+        const PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
+        const PairC c1 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q1.w));
+        pair_term<STYLE, EV>(a, pi, vi, rho_i, q0, w0, r0, e0 >> IMG_SHIFT, c0, dtis, d);
+        pair_term<STYLE, EV>(a, pi, vi, rho_i, q1, w1, r1, e1 >> IMG_SHIFT, c1, dtis, d);
+    }
+    if (t < cnt) {
+        const uint32_t e0 = (uint32_t)crow[(size_t)(t * SPLIT + w) * WAVE];
+        const uint32_t j0 = e0 & IDX_MASK;
+        const float4 q0 = pos[j0];
+        const float r0 = STYLE == STYLE_MDPD ? P.rho[j0] : 0.f;
+        const PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
+        pair_term<STYLE, EV>(a, pi, vi, rho_i, q0, vel[j0], r0, e0 >> IMG_SHIFT, c0, dtis, d);
+    }
+    if (SPLIT > 1) {
+        if (w > 0)
+            for (int k = 0; k < NA; ++k) s_acc[w - 1][k][lane] = a.v[k];
+        __syncthreads();
+        if (w == 0)
+            for (int q = 0; q < SPLIT - 1; ++q)
+                for (int k = 0; k < NA; ++k) a.v[k] += s_acc[q][k][lane];
+    }
     int moved = 0;
-    if (i < d->part_base + d->n) {
-        const int cur = d->sel;
-        const float4 *pos = P.pos[cur];
-        const float4 *vel = P.vel[cur];
-        const float4 pi = pos[i];
-        const float4 vi = vel[i];
-        const uint32_t sig_i = __float_as_uint(vi.w);
-        const int ti = __float_as_int(pi.w);
-        const float rho_i = STYLE == STYLE_MDPD ? P.rho[i] : 0.f;
-        const float dtis = d->dtinvsqrt;
-        const float *coef = P.coef + d->coef_base;
-        const int nt = d->ntypes;
-        Pair6 cu;
-        if (UNI) cu = load_coef(coef, 0);
-        const int maxnbr = P.maxnbr;
-        const int cnt = min(P.ncount[i], maxnbr);
-        const int *row = P.nlist + (size_t)(i >> 6) * maxnbr * WAVE + (i & 63);
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        float en = 0.f, vxx = 0.f, vyy = 0.f, vzz = 0.f, vxy = 0.f, vxz = 0.f, vyz = 0.f;
-        for (int p = 0; p < cnt; ++p) {
-            const uint32_t e = (uint32_t)row[(size_t)p * WAVE];
-            const uint32_t j = e & IDX_MASK, code = e >> IMG_SHIFT;
-            const float4 q = pos[j];
-            float xj = q.x, yj = q.y, zj = q.z;
-            if (code != IMG_ZERO) apply_image(code, d, xj, yj, zj);
-            const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-            const float rsq = dx * dx + dy * dy + dz * dz;
-            const Pair6 c = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q.w));
-            if (rsq < c.rc2 && rsq >= 1e-20f) {
-                const float4 vj = vel[j];
-                const float rinv = __builtin_amdgcn_rsqf(rsq);
-                const float r = rsq * rinv;
-                const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
-                const float dot = dx * dvx + dy * dvy + dz * dvz;
-                const float wc = 1.f - r * c.rcinv;
-                const float xi = pair_gaussian(sig_i, __float_as_uint(vj.w));
-                float fpair;
-                if (STYLE == STYLE_MDPD) {
-                    // pair_mdpd_meso.cu:156-191
-                    const float wcr = fmaxf(1.f - r * c.rdinv, 0.f);
-                    const float rr = rho_i + P.rho[j];
-                    fpair = (c.a * wc + c.b * rr * wcr) - c.gamma * wc * wc * dot * rinv + c.sigma * wc * xi * dtis;
-                    fpair *= rinv;
-                    if (EV) en += 0.5f * c.a * (1.f / c.rcinv) * wc * wc + 0.5f * c.b * (1.f / c.rdinv) * rr * wcr * wcr;
-                } else {
-                    // pair_dpd_meso.cu:145-175
-                    const float wr = c.expw == 1.f ? wc : powf(wc, c.expw);
-                    fpair = c.a * wc - c.gamma * wr * wr * dot * rinv + c.sigma * wr * xi * dtis;
-                    fpair *= rinv;
-                    if (EV) en += 0.5f * c.a * (1.f / c.rcinv) * wc * wc;
-                }
-                fx += dx * fpair; fy += dy * fpair; fz += dz * fpair;
-                if (EV) {
-                    vxx += dx * dx * fpair; vyy += dy * dy * fpair; vzz += dz * dz * fpair;
-                    vxy += dx * dy * fpair; vxz += dx * dz * fpair; vyz += dy * dz * fpair;
-                }
-            }
-        }
+    if (w == 0 && valid) {
+        float fx = a.v[0], fy = a.v[1], fz = a.v[2];
         const int mk = P.mask[d->sela][i];
         if (mk & d->add_bit) { fx += d->add[0]; fy += d->add[1]; fz += d->add[2]; }
         if (EV) {
-            P.frc[i] = make_float4(fx, fy, fz, 0.5f * en);
+            P.frc[i] = make_float4(fx, fy, fz, 0.5f * a.v[3]);
             const size_t cap = (size_t)P.cap;
-            P.vir[i] = 0.5f * vxx; P.vir[cap + i] = 0.5f * vyy; P.vir[2 * cap + i] = 0.5f * vzz;
-            P.vir[3 * cap + i] = 0.5f * vxy; P.vir[4 * cap + i] = 0.5f * vxz; P.vir[5 * cap + i] = 0.5f * vyz;
+            P.vir[i] = 0.5f * a.v[4]; P.vir[cap + i] = 0.5f * a.v[5]; P.vir[2 * cap + i] = 0.5f * a.v[6];
+            P.vir[3 * cap + i] = 0.5f * a.v[7]; P.vir[4 * cap + i] = 0.5f * a.v[8]; P.vir[5 * cap + i] = 0.5f * a.v[9];
         }
         if (MODE != MODE_SETUP) {
             float vx = vi.x, vy = vi.y, vz = vi.z;
@@ -291,10 +370,33 @@ __global__ void __launch_bounds__(BLK) k_force(PoolDev P, int nlogical)
             }
         }
     }
-    if (MODE == MODE_FUSED || MODE == MODE_FUSED_OUT)
-        block_commit(d, moved, [&]() { commit_advance(d); });
-    else if (MODE == MODE_FINAL)
-        block_commit(d, 0, [&]() { d->sel ^= 1; });
+    if (MODE == MODE_FUSED || MODE == MODE_FUSED_OUT) tile_flags(P.moved, tile, 1, moved);
+}
+
+// --------------------------------------------------------------------------
+// Step commit, one workgroup per system, after a pass that wrote the other
+// buffer: ADVANCE (initial kick or fused step) flips the buffer, advances the
+// step, precomputes the next TEA seed and ORs the tiles' moved flags into
+// `stale`; FINAL (end-of-run half-kick) only flips the buffer.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance)
+{
+    __shared__ int s_any;
+    SysDev *d = P.sys + blockIdx.x;
+    if (!d->active) return;
+    if (threadIdx.x == 0) s_any = 0;
+    __syncthreads();
+    if (advance) {
+        const int t0 = d->part_base / WAVE, nt = d->npad / WAVE;
+        int any = 0;
+        for (int t = threadIdx.x; t < nt; t += blockDim.x) any |= P.moved[t0 + t];
+        if (any) s_any = 1;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (advance) commit_advance(d, s_any != 0);
+        else d->sel ^= 1;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -318,11 +420,32 @@ __global__ void k_premix(const uint32_t *a, const uint32_t *b, uint32_t *out, in
 // ----------------------------------------------------------------------------
 // launchers
 // ----------------------------------------------------------------------------
-hipError_t launch_density(const PoolDev &P, hipStream_t st)
+template <int SPLIT>
+static void launch_first_t(const PoolDev &P, bool with_rho, int nt, hipStream_t st)
 {
-    const int nl = P.cap / BLK;
-    if (nl == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_density, dim3(xcd_grid(nl)), dim3(BLK), 0, st, P, nl);
+    if (with_rho)
+        hipLaunchKernelGGL((k_first<true, SPLIT>), dim3(xcd_grid(nt)), dim3(WAVE * SPLIT), 0, st, P, nt);
+    else
+        hipLaunchKernelGGL((k_first<false, SPLIT>), dim3(xcd_grid(nt)), dim3(WAVE * SPLIT), 0, st, P, nt);
+}
+
+hipError_t launch_first(const PoolDev &P, bool with_rho, int split, hipStream_t st)
+{
+    const int nt = P.cap / WAVE;
+    if (nt == 0) return hipSuccess;
+    switch (split) {
+    case 1: launch_first_t<1>(P, with_rho, nt, st); break;
+    case 2: launch_first_t<2>(P, with_rho, nt, st); break;
+    case 4: launch_first_t<4>(P, with_rho, nt, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_commit_step(const PoolDev &P, bool advance, hipStream_t st)
+{
+    if (P.nsys == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_commit_step, dim3(P.nsys), dim3(256), 0, st, P, advance ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -334,21 +457,33 @@ hipError_t launch_initial(const PoolDev &P, hipStream_t st)
     return hipGetLastError();
 }
 
-template <int STYLE, int MODE>
-static void launch_force_t(const PoolDev &P, bool uni, int nl, hipStream_t st)
+template <int STYLE, int MODE, int SPLIT>
+static void launch_force_t(const PoolDev &P, bool uni, int nt, hipStream_t st)
 {
     if (uni)
-        hipLaunchKernelGGL((k_force<STYLE, MODE, true>), dim3(xcd_grid(nl)), dim3(BLK), 0, st, P, nl);
+        hipLaunchKernelGGL((k_force<STYLE, MODE, true, SPLIT>), dim3(xcd_grid(nt)), dim3(WAVE * SPLIT), 0, st, P, nt);
     else
-        hipLaunchKernelGGL((k_force<STYLE, MODE, false>), dim3(xcd_grid(nl)), dim3(BLK), 0, st, P, nl);
+        hipLaunchKernelGGL((k_force<STYLE, MODE, false, SPLIT>), dim3(xcd_grid(nt)), dim3(WAVE * SPLIT), 0, st, P, nt);
 }
 
-hipError_t launch_force(const PoolDev &P, int style, int mode, bool uni, hipStream_t st)
+template <int STYLE, int MODE>
+static hipError_t launch_force_s(const PoolDev &P, bool uni, int split, int nt, hipStream_t st)
 {
-    const int nl = P.cap / BLK;
-    if (nl == 0) return hipSuccess;
+    switch (split) {
+    case 1: launch_force_t<STYLE, MODE, 1>(P, uni, nt, st); break;
+    case 2: launch_force_t<STYLE, MODE, 2>(P, uni, nt, st); break;
+    case 4: launch_force_t<STYLE, MODE, 4>(P, uni, nt, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_force(const PoolDev &P, int style, int mode, bool uni, int split, hipStream_t st)
+{
+    const int nt = P.cap / WAVE;
+    if (nt == 0) return hipSuccess;
 #define MDPD_FORCE_CASE(S, M) \
-    if (style == S && mode == M) { launch_force_t<S, M>(P, uni, nl, st); return hipGetLastError(); }
+    if (style == S && mode == M) return launch_force_s<S, M>(P, uni, split, nt, st);
     MDPD_FORCE_CASE(STYLE_MDPD, MODE_SETUP)
     MDPD_FORCE_CASE(STYLE_MDPD, MODE_FUSED)
     MDPD_FORCE_CASE(STYLE_MDPD, MODE_FUSED_OUT)

@@ -83,6 +83,7 @@ def load():
         "mdpd_gather_atoms": (I, [P, ctypes.c_char_p, I, I, P]),
         "mdpd_scatter_atoms": (I, [P, ctypes.c_char_p, I, I, P]),
         "mdpd_thermo": (I, [P, ctypes.POINTER(D)]),
+        "mdpd_tune": (I, [P, ctypes.c_char_p, I]),
         "mdpd_profile": (I, [P, I]),
         "mdpd_profile_get": (I, [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(LL)]),
         "mdpd_debug_sorted": (I, [P, P, P, P]),
@@ -206,7 +207,10 @@ class System:
         self._check(self._lib.mdpd_thermo(self._h, out))
         return {"temp": out[0], "press": out[1], "pe": out[2], "ke": out[3]}
 
-    # -- kernel timing --
+    # -- tuning / kernel timing --
+    def tune(self, knob, value):
+        self._check(self._lib.mdpd_tune(self._h, knob.encode(), int(value)))
+
     def profile(self, enable=True):
         self._check(self._lib.mdpd_profile(self._h, int(bool(enable))))
 
