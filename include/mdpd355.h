@@ -151,6 +151,63 @@ int mdpd_tea_gaussian(int device, int n, const unsigned int *a, const unsigned i
 int mdpd_tea_premix(int device, int n, const unsigned int *a, const unsigned int *b, int rounds,
                     unsigned int *out);
 
+/* ---- pooled engine (batched and served runs) -------------------------------
+ * Many independent systems ("jobs") packed into one set of device arrays and
+ * advanced together, one launch sequence per pass for the whole pool.  The
+ * reference's closest counterpart is its ensemble mode, which runs
+ * independent cases as separate MPI communicators (src/main.cpp:49-117), and
+ * each job follows ModifiedVerlet::run (src/USER-MESO/mvv_meso.cu:309-572)
+ * exactly as mdpd_run does on a handle of its own, with the reneighbor
+ * decision (neigh_modify every/delay/check, src/neighbor.cpp:1216-1290) made
+ * per job on the device.
+ *
+ * A job is submitted from a configured handle (script state + atoms are
+ * copied; the handle stays the caller's and is not changed), is admitted when
+ * its particles and cells fit the pool's HBM budget (first in, first out),
+ * keeps its state resident while it runs, and leaves when done; its results
+ * stay readable until mdpd_pool_release.  One host thread drives a pool. */
+typedef struct mdpd_pool mdpd_pool;
+
+enum { MDPD_JOB_QUEUED = 0, MDPD_JOB_RUNNING = 1, MDPD_JOB_DONE = 2 };
+
+/* hbm_budget_bytes: device memory the pool may hold (its particle capacity is
+ * derived from it at the first admission); max_systems: jobs resident at once. */
+int mdpd_pool_open(int device, double hbm_budget_bytes, int max_systems, mdpd_pool **out);
+void mdpd_pool_close(mdpd_pool *pool);
+const char *mdpd_pool_last_error(const mdpd_pool *pool);
+/* Queue `sys` for nsteps steps (like `run nsteps` on it); *job = its id. */
+int mdpd_pool_submit(mdpd_pool *pool, mdpd_system *sys, long long nsteps, int *job);
+/* Drive the device: admit queued jobs, run passes, harvest finished jobs.
+ * Returns after max_passes passes (0: no limit), as soon as a job finished
+ * when stop_on_done, or when no job is left; *ndone = jobs finished. */
+int mdpd_pool_advance(mdpd_pool *pool, long long max_passes, int stop_on_done, int *ndone);
+/* Run until every submitted job is done. */
+int mdpd_pool_run(mdpd_pool *pool);
+int mdpd_pool_job_state(mdpd_pool *pool, int job, int *state);
+/* Ids of jobs finished since the last call (up to max). */
+int mdpd_pool_done(mdpd_pool *pool, int *jobs, int max, int *count);
+/* Per-atom results of a finished job, creation order, as mdpd_gather_atoms:
+ * x, v, f (type 1, count 3), e_pair, rho (1, 1), virial (1, 6), tag, type,
+ * mask (0, 1), image (0, 3).  f, e_pair and virial are those of the last
+ * force evaluation, as after `run` on a handle. */
+int mdpd_pool_gather(mdpd_pool *pool, int job, const char *name, int type, int count, void *data);
+/* {T, P, pe, ke} of a finished job, as mdpd_thermo. */
+int mdpd_pool_thermo(mdpd_pool *pool, int job, double out[4]);
+/* Atoms, final timestep, and submit / admit / done times (seconds on
+ * mdpd_clock) of a job. */
+int mdpd_pool_job_info(mdpd_pool *pool, int job, long long *natoms, long long *timestep, double times[3]);
+int mdpd_pool_release(mdpd_pool *pool, int job);
+/* "split" 0 (autotune on the first passes, the default), 1, 2 or 4;
+ * "graph" 1/0: replay each pass as a captured hipGraph (default 1);
+ * "poll": passes between status reads (default 8). */
+int mdpd_pool_tune(mdpd_pool *pool, const char *knob, int value);
+/* Statistics: passes, particle_steps (of finished jobs), split, capacity
+ * (particle slots), maxnbr, bytes (device bytes held), regrows, running,
+ * queued, used_slots, tune_ms_1 / tune_ms_2 / tune_ms_4 (ms per pass). */
+int mdpd_pool_stat(mdpd_pool *pool, const char *name, double *value);
+/* Monotonic clock of the job times, in seconds. */
+double mdpd_clock(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,3 +125,57 @@ def test_open_does_not_need_a_device(engine):
     s = engine.System(device=0)
     s.command("units lj")
     s.close()
+
+
+def _job(engine, n=500, L=8.0, seed=1):
+    s = engine.System()
+    _deck(s, L)
+    rng = np.random.default_rng(seed)
+    s.create_atoms(np.ones(n, np.int32), rng.uniform(0, L, (n, 3)))
+    s.commands("""
+        mass 1 1.0
+        pair_style mdpd/meso 1.0 1.0 123456
+        pair_coeff * * -40 25 4.5 1.0 0.75
+        fix rho all aboundary/rho/meso cut_rho 0.75
+        fix int all nve/meso
+        timestep 0.01
+        run_style mvv/meso
+    """)
+    return s
+
+
+def test_pool_queue_and_errors_without_a_device(engine):
+    # submission validates and copies the system on the host; the device is
+    # first touched when the pool is advanced
+    p = engine.Pool(64e6, 4)
+    j = p.submit(_job(engine), 10)
+    assert p.state(j) == engine.JOB_QUEUED
+    assert p.stat("queued") == 1 and p.stat("running") == 0
+    # 64 MB at 256 + 8*112 bytes per slot
+    assert 50000 < p.stat("capacity") < 64e6 / 1000
+    with pytest.raises(engine.MdpdError) as e:
+        p.gather(j, "x")
+    assert e.value.code == 2
+    with pytest.raises(engine.MdpdError) as e:
+        p.release(j)
+    assert e.value.code == 2
+    with pytest.raises(engine.MdpdError) as e:
+        p.submit(_job(engine), 0)
+    assert e.value.code == 1
+    # a job larger than the whole budget is refused at submission
+    small = engine.Pool(2e6, 4)
+    with pytest.raises(engine.MdpdError) as e:
+        small.submit(_job(engine, n=4000, L=9.0), 10)
+    assert e.value.code == 5
+    # an incomplete script is refused before it reaches the queue
+    s = engine.System()
+    _deck(s)
+    s.create_atoms(np.ones(10, np.int32), np.random.default_rng(0).uniform(0, 8, (10, 3)))
+    s.command("run_style mvv/meso")
+    with pytest.raises(engine.MdpdError) as e:
+        p.submit(s, 10)
+    assert e.value.code == 2
+    with pytest.raises(engine.MdpdError):
+        p.tune("split", 3)
+    with pytest.raises(engine.MdpdError):
+        p.stat("nonsense")

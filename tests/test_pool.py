@@ -1,0 +1,151 @@
+"""Pooled engine (batched / served runs) through the C ABI.
+
+A job in the pool runs the same kernels, in the same order, as a plain run
+of the same system (`run N` on its own handle): the per-system results are
+bit-identical to the plain run and to the same job run alone in a pool, also
+when it is admitted late into the slot and ranges a finished job freed, and
+when the pool widens its neighbor rows mid-run.  Only a different launch
+split (waves per tile) may change results, by FP32 summation order; that case
+is held to 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import uniform_box
+from test_gpu_parity import assert_vec_close, dpd_system, mdpd_system
+
+pytestmark = pytest.mark.gpu
+
+GB = float(1 << 30)
+
+
+def droplet(engine, n, seed, rho=6.05, extra="", rho_est=6.05, ntypes=1):
+    """n particles uniform in a sphere at density rho, cube side 2.5 R."""
+    rng = np.random.default_rng(seed)
+    R = (3.0 * n / (4.0 * np.pi * rho)) ** (1.0 / 3.0)
+    L = 2.5 * R
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r = R * rng.uniform(0, 1, n) ** (1.0 / 3.0)
+    x = (L / 2 + d * r[:, None]).astype(np.float32).astype(np.float64)
+    v = rng.normal(0, 1, (n, 3))
+    v = (v - v.mean(axis=0)).astype(np.float32).astype(np.float64)
+    types = None if ntypes == 1 else rng.integers(1, ntypes + 1, n).astype(np.int32)
+    coeff = "* * -40 25 4.5 1.0 0.75"
+    if ntypes == 2:
+        coeff = "1 1 -40 25 4.5 1.0 0.75\npair_coeff 1 2 -30 25 4.5 1.0 0.75\npair_coeff 2 2 -40 25 4.5 1.0 0.75"
+    return mdpd_system(engine, x, v, np.array([L, L, L]), types=types, ntypes=ntypes, extra=extra,
+                       rho_est=rho_est, seed=1000 + seed, coeff=coeff)
+
+
+def plain_result(s, nsteps):
+    s.tune("split", 2)
+    s.run(nsteps)
+    return {k: s.gather(k) for k in ("x", "v", "f", "image", "e_pair", "virial")}, s.thermo(), s.timestep
+
+
+def pool_results(engine, systems, nsteps, budget=0.5 * GB, max_systems=16, split=2, poll=4):
+    p = engine.Pool(budget, max_systems)
+    p.tune("split", split)
+    p.tune("poll", poll)
+    ids = [p.submit(s, k) for s, k in zip(systems, nsteps)]
+    p.run()
+    out = []
+    for j in ids:
+        assert p.state(j) == engine.JOB_DONE
+        r = {k: p.gather(j, k) for k in ("x", "v", "f", "image", "e_pair", "virial")}
+        out.append((r, p.thermo(j), p.info(j)["timestep"]))
+    return out, p
+
+
+def assert_same(a, b, what):
+    for k in a[0]:
+        np.testing.assert_array_equal(a[0][k], b[0][k], err_msg=f"{what}: {k}")
+    assert a[2] == b[2], (what, a[2], b[2])
+    for k in a[1]:
+        assert a[1][k] == b[1][k], (what, k, a[1][k], b[1][k])
+
+
+def test_single_job_equals_plain_run(engine):
+    n, L = 1296, 6.0
+    x, v = uniform_box(n, L, 11)
+    ref = plain_result(mdpd_system(engine, x, v, L), 25)
+    got, p = pool_results(engine, [mdpd_system(engine, x, v, L)], [25])
+    assert_same(got[0], ref, "single mDPD job")
+    assert p.stat("passes") >= 26
+
+
+def test_mixed_pool_equals_each_alone(engine):
+    # mDPD droplets of several sizes, one with two types and a body force,
+    # a DPD cube; different run lengths
+    systems = lambda: [
+        droplet(engine, 2000, 1),
+        droplet(engine, 5000, 2, ntypes=2, extra="fix push all addforce/meso 0.05 0 0"),
+        dpd_system(engine, *uniform_box(3000, 10.0, 3), 10.0),
+        droplet(engine, 800, 4),
+    ]
+    nsteps = [30, 17, 24, 9]
+    alone = [pool_results(engine, [s], [k])[0][0] for s, k in zip(systems(), nsteps)]
+    mixed, p = pool_results(engine, systems(), nsteps)
+    for k, (a, b) in enumerate(zip(mixed, alone)):
+        assert_same(a, b, f"job {k}")
+    # and each equals a plain run of the same system
+    plain = plain_result(systems()[1], nsteps[1])
+    assert_same(mixed[1], plain, "two-type droplet vs plain")
+
+
+def test_queued_jobs_reuse_freed_ranges(engine):
+    # a budget that holds about two of these jobs: the rest wait in the queue
+    # and are admitted into the slot / ranges of finished ones
+    mk = lambda: [droplet(engine, 3000 + 500 * k, 20 + k) for k in range(6)]
+    nsteps = [12, 20, 8, 15, 10, 6]
+    alone = [pool_results(engine, [s], [k])[0][0] for s, k in zip(mk(), nsteps)]
+    small, p = pool_results(engine, mk(), nsteps, budget=9.0e6, max_systems=3)
+    assert p.stat("capacity") < 3 * 6000
+    for k, (a, b) in enumerate(zip(small, alone)):
+        assert_same(a, b, f"queued job {k}")
+
+
+def test_narrow_rows_regrow_in_place(engine):
+    # rho_est 0.5 sizes the rows for ~9 neighbors: the first build overflows,
+    # the pool widens its rows (no re-sort) and the results do not change
+    n, L = 1296, 6.0
+    x, v = uniform_box(n, L, 12)
+    ref = plain_result(mdpd_system(engine, x, v, L), 15)
+    got, p = pool_results(engine, [mdpd_system(engine, x, v, L, rho_est=0.5)], [15])
+    assert p.stat("regrows") >= 1
+    assert_same(got[0], ref, "regrown rows")
+
+
+def test_autotuned_split_within_tolerance(engine):
+    n, L = 1296, 6.0
+    x, v = uniform_box(n, L, 13)
+    ref = plain_result(mdpd_system(engine, x, v, L), 5)
+    got, p = pool_results(engine, [mdpd_system(engine, x, v, L)], [5], split=0)
+    assert p.stat("split") in (1, 2, 4)
+    assert p.stat("tune_ms_2") > 0
+    assert_vec_close(got[0][0]["x"] - x, ref[0]["x"] - x, 1e-5, "autotuned displacement")
+    assert_vec_close(got[0][0]["f"], ref[0]["f"], 1e-5, "autotuned force")
+
+
+def test_served_submission_while_running(engine):
+    # jobs submitted while others are on the device (continuous batching):
+    # results equal each job alone
+    mk = [lambda k=k: droplet(engine, 1500 + 700 * k, 40 + k) for k in range(5)]
+    nsteps = [40, 10, 25, 5, 30]
+    alone = [pool_results(engine, [m()], [k])[0][0] for m, k in zip(mk, nsteps)]
+    p = engine.Pool(0.5 * GB, 8)
+    p.tune("split", 2)
+    p.tune("poll", 2)
+    ids = [p.submit(mk[0](), nsteps[0])]
+    for k in range(1, 5):
+        p.advance(max_passes=6)
+        ids.append(p.submit(mk[k](), nsteps[k]))
+    p.run()
+    for k, j in enumerate(ids):
+        r = ({q: p.gather(j, q) for q in ("x", "v", "f", "image", "e_pair", "virial")}, p.thermo(j),
+             p.info(j)["timestep"])
+        assert_same(r, alone[k], f"served job {k}")
+    info = [p.info(j) for j in ids]
+    assert all(i["t_submit"] <= i["t_admit"] <= i["t_done"] for i in info)
+    assert sorted(p.done()) == sorted(ids)

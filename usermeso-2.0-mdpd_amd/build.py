@@ -18,8 +18,8 @@ OUT_DIR = os.path.join(HERE, "mdpd355")
 LIB = os.path.join(OUT_DIR, "libmdpd355.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["kernels_build.hip", "kernels_step.hip", "engine.cpp"]
-HEADERS = ["common.hpp", "kernels.hpp"]
+SOURCES = ["kernels_build.hip", "kernels_step.hip", "engine.cpp", "pool.cpp"]
+HEADERS = ["common.hpp", "kernels.hpp", "system.hpp"]
 
 ARCH = "gfx950"
 FLAGS = [

@@ -17,6 +17,11 @@ constexpr int IMG_ZERO = 13;       // (0,0,0) image: (sx+1) + 3(sy+1) + 9(sz+1)
 
 enum Style : int { STYLE_MDPD = 0, STYLE_DPD = 1 };
 
+// Lifecycle of a system in a pooled (batched / served) launch.  A job of K
+// steps makes K+1 force passes: FRESH (setup forces + first half-kick and
+// drift), K-1 RUN passes (fused step), one LAST pass (final half-kick only).
+enum SysPhase : int { PH_FRESH = 0, PH_RUN = 1, PH_LAST = 2, PH_DONE = 3 };
+
 // Per type pair coefficients in FP32, laid out as NCOEF floats.
 // mDPD (pair_mdpd_meso.cu:70-91): rc, 1/rc, rd, 1/rd, A, B, gamma, sigma.
 // DPD  (pair_dpd_meso.cu):        rc, 1/rc, -, -,   a0, -, gamma, sigma, expw.
@@ -58,11 +63,17 @@ struct SysDev {
     int stale_pending;           // set during a step by the displacement check
     int overflow;                // a neighbor row or cell exceeded its capacity
     int active;                  // 0: slot idle (batched / served engine)
-    int steps_left;              // served engine: steps still to run
+    int steps_left;              // pooled engine: steps still to run
     long long step;              // ntimestep of the next / current force evaluation
-    unsigned long long done;     // last-workgroup commit: arrivals (low 32) + moved flags (high 32)
     uint32_t seed_next;          // premix_TEA<64>(seed, step + 1)
-    int pad_;
+    uint32_t seed_cur;           // premix_TEA<64>(seed, step): signatures of a fresh system
+    int phase;                   // pooled engine: PH_FRESH / PH_RUN / PH_LAST / PH_DONE
+    int job;                     // pooled engine: caller's job id of this slot
+    int nne;                     // non-empty cells at the last re-sort
+    int split_last;              // pooled engine: waves per tile of the system's last pass
+    // pooled engine: Neighbor::decide on the device (src/neighbor.cpp:1216-1290)
+    int ago;                     // steps since the last build
+    int n_every, n_delay, n_check;
 };
 
 // Device arrays of one pool (a pool packs one or more systems).
@@ -92,6 +103,10 @@ struct PoolDev {
     int *ccount;         // per particle
     int *wave_sys;       // system of each 64-particle wave
     int *moved;          // per 64-particle tile: a particle moved past skin/2 this step
+    int *status;         // pooled engine, per system: job << 3 | 1 stale | 2 done | 4 overflow
+    int *cl;             // per particle slot: a system's non-empty cells (staging)
+    int *wl;             // rebuild work list: non-empty cell slots of the systems being rebuilt
+    int *work_total;     // entries in wl
     SysDev *sys;
     float *coef;         // pooled coefficient tables
     float *minv;         // pooled per-type 1/m

@@ -10,11 +10,15 @@ enum ForceMode : int {
     MODE_FUSED = 1,      // force, final half-kick, next initial half-kick + drift
     MODE_FUSED_OUT = 2,  // MODE_FUSED and also write f, e, virial, end-of-step v
     MODE_FINAL = 3,      // force and final half-kick only (end of a run)
+    MODE_PHASED = 4,     // pooled engine: each system follows its own SysPhase
+    MODE_HARVEST = 5,    // pooled engine, finished system: f, e, virial of its last pass
 };
 
 // rebuild phase: wrap, keys, sort, gather, neighbor rows (stale systems only)
 hipError_t launch_rebuild(const PoolDev &P, hipStream_t st);
 hipError_t launch_nlist_only(const PoolDev &P, hipStream_t st);
+// neighbor rows of every sorted active system rebuilt in place (row width grew)
+hipError_t launch_relist(const PoolDev &P, hipStream_t st);
 hipError_t launch_commit_rebuild(const PoolDev &P, hipStream_t st);
 hipError_t launch_signatures(const PoolDev &P, hipStream_t st);
 
@@ -23,8 +27,14 @@ hipError_t launch_signatures(const PoolDev &P, hipStream_t st);
 hipError_t launch_first(const PoolDev &P, bool with_rho, int split, hipStream_t st);
 hipError_t launch_initial(const PoolDev &P, hipStream_t st);
 // after launch_initial / a fused force pass (advance) or a final pass (!advance)
-hipError_t launch_commit_step(const PoolDev &P, bool advance, hipStream_t st);
+// kind 0 final, 1 advance, 2 phased (pooled); split = waves per tile of the pass
+hipError_t launch_commit_step(const PoolDev &P, int kind, int split, hipStream_t st);
 hipError_t launch_force(const PoolDev &P, int style, int mode, bool uni, int split, hipStream_t st);
+// the same over the 64-particle tiles [tile0, tile0 + ntiles) only
+hipError_t launch_force_range(const PoolDev &P, int style, int mode, bool uni, int split, int tile0, int ntiles,
+                              hipStream_t st);
+// pooled engine, finished system s: current data into buffer 0 (sel = sela = 0)
+hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, hipStream_t st);
 
 // stateless helpers for parity tests
 hipError_t launch_pair_gaussian(const uint32_t *a, const uint32_t *b, float *out, int n, hipStream_t st);

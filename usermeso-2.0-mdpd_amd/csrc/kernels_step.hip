@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
     const int s = P.wave_sys[tile];
     if (s < 0) return;
     const SysDev *d = P.sys + s;
-    if (!d->active) return;
+    if (!d->active || d->overflow) return;    // an overflowed system waits for wider rows
     const bool valid = i < d->part_base + d->n;
     const float4 *pos = P.pos[d->sel];
     const int maxnbr = P.maxnbr;
@@ -266,22 +266,31 @@ __device__ inline void pair_term(Acc<EV> &a, const float4 &pi, const float4 &vi,
 }
 
 template <int STYLE, int MODE, bool UNI, int SPLIT>
-__global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int ntiles)
+__global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, int ntiles)
 {
-    constexpr bool EV = MODE != MODE_FUSED;
+    // HARVEST re-evaluates the last pass of a finished pooled system from the
+    // state that pass read, which its commit left in the other buffer; it
+    // adds energy and virial and changes no state
+    constexpr bool EV = MODE != MODE_FUSED && MODE != MODE_PHASED;
+    constexpr bool STEP = MODE != MODE_SETUP && MODE != MODE_HARVEST;
     constexpr int NA = Acc<EV>::N;
     __shared__ float s_acc[SPLIT > 1 ? SPLIT - 1 : 1][NA][WAVE];
-    const int tile = xcd_logical(blockIdx.x, ntiles);
-    if (tile >= ntiles) return;
+    const int lt = xcd_logical(blockIdx.x, ntiles);
+    if (lt >= ntiles) return;
+    const int tile = tile0 + lt;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = tile * WAVE + lane;
     const int s = P.wave_sys[tile];
     if (s < 0) return;
     SysDev *d = P.sys + s;
-    if (!d->active) return;
+    if (!d->active && MODE != MODE_HARVEST) return;
     if (UNI != (d->ntypes == 1)) return;     // host launches the matching variant
+    // a pool may mix styles: each launch takes the systems of its style, and
+    // a system waiting for wider neighbor rows is skipped
+    if (MODE == MODE_PHASED && (d->style != STYLE || d->overflow)) return;
+    if (MODE == MODE_HARVEST && d->split_last != SPLIT) return;   // the split of its last pass
     const bool valid = i < d->part_base + d->n;
-    const int cur = d->sel;
+    const int cur = MODE == MODE_HARVEST ? d->sel ^ 1 : d->sel;
     const float4 *pos = P.pos[cur];
     const float4 *vel = P.vel[cur];
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
@@ -343,17 +352,21 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int ntiles)
             P.vir[i] = 0.5f * a.v[4]; P.vir[cap + i] = 0.5f * a.v[5]; P.vir[2 * cap + i] = 0.5f * a.v[6];
             P.vir[3 * cap + i] = 0.5f * a.v[7]; P.vir[4 * cap + i] = 0.5f * a.v[8]; P.vir[5 * cap + i] = 0.5f * a.v[9];
         }
-        if (MODE != MODE_SETUP) {
+        if (STEP) {
+            // which half-kicks this pass applies: the pooled engine decides per
+            // system, the plain engine per launch
+            const int ph = MODE == MODE_PHASED ? d->phase : (MODE == MODE_FINAL ? (int)PH_LAST : (int)PH_RUN);
             float vx = vi.x, vy = vi.y, vz = vi.z;
             float4 pn = pi;
-            if (mk & d->nve_bit) {
+            if ((mk & d->nve_bit) && ph != PH_FRESH) {
                 const float dtfm = d->dtf * P.minv[d->mass_base + ti];
                 vx += dtfm * fx; vy += dtfm * fy; vz += dtfm * fz;             // final_integrate
             }
-            if (MODE == MODE_FINAL) {
+            if (ph == PH_LAST) {
                 P.pos[cur ^ 1][i] = pi;
                 P.vel[cur ^ 1][i] = make_float4(vx, vy, vz, vi.w);
                 P.vend[i] = make_float4(vx, vy, vz, 0.f);
+                if (MODE == MODE_PHASED) P.frc[i] = make_float4(fx, fy, fz, 0.f);
             } else {
                 if (MODE == MODE_FUSED_OUT) P.vend[i] = make_float4(vx, vy, vz, 0.f);
                 if (mk & d->nve_bit) {
@@ -370,20 +383,29 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int ntiles)
             }
         }
     }
-    if (MODE == MODE_FUSED || MODE == MODE_FUSED_OUT) tile_flags(P.moved, tile, 1, moved);
+    if (MODE == MODE_FUSED || MODE == MODE_FUSED_OUT || MODE == MODE_PHASED) tile_flags(P.moved, tile, 1, moved);
 }
 
 // --------------------------------------------------------------------------
 // Step commit, one workgroup per system, after a pass that wrote the other
 // buffer: ADVANCE (initial kick or fused step) flips the buffer, advances the
 // step, precomputes the next TEA seed and ORs the tiles' moved flags into
-// `stale`; FINAL (end-of-run half-kick) only flips the buffer.
+// `stale`; FINAL (end-of-run half-kick) only flips the buffer.  PHASED (pooled
+// engine) follows each system's lifecycle and makes the reneighbor decision
+// of Neighbor::decide (src/neighbor.cpp:1216-1290) on the device: after
+// `ago` steps, rebuild when ago >= delay, ago % every == 0 and (no check, or
+// a particle moved more than skin/2 since the last build).
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance)
+__global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance, int split)  // 0 final, 1 advance, 2 phased
 {
     __shared__ int s_any;
     SysDev *d = P.sys + blockIdx.x;
     if (!d->active) return;
+    if (advance == 2 && d->overflow) {
+        // skipped by the passes: no state change until the rows are regrown
+        if (threadIdx.x == 0) P.status[blockIdx.x] = (d->job << 3) | (d->stale ? 1 : 0) | 4;
+        return;
+    }
     if (threadIdx.x == 0) s_any = 0;
     __syncthreads();
     if (advance) {
@@ -394,8 +416,50 @@ __global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance)
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (advance) commit_advance(d, s_any != 0);
-        else d->sel ^= 1;
+        if (advance == 2) {
+            // pooled engine: per-system lifecycle
+            d->split_last = split;
+            if (d->phase == PH_LAST) {
+                d->sel ^= 1;
+                d->phase = PH_DONE;
+                d->active = 0;
+            } else {
+                commit_advance(d, false);
+                const int ago = ++d->ago;
+                const bool due = ago >= d->n_delay && ago % d->n_every == 0;
+                d->stale = due && (!d->n_check || s_any != 0) ? 1 : 0;
+                d->steps_left -= 1;
+                d->phase = d->steps_left <= 0 ? PH_LAST : PH_RUN;
+            }
+            P.status[blockIdx.x] = (d->job << 3) | (d->stale ? 1 : 0) | (d->phase == PH_DONE ? 2 : 0);
+        } else if (advance) {
+            commit_advance(d, s_any != 0);
+        } else {
+            d->sel ^= 1;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Finished pooled system: its current particle data into buffer 0, so the
+// host copies one contiguous range per array.  Runs after the HARVEST force
+// pass, which reads the other buffer.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_harvest(PoolDev P, int s, int pb, int npad)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npad) return;
+    const SysDev *d = P.sys + s;
+    const int i = pb + k;
+    if (d->sel) {
+        P.pos[0][i] = P.pos[1][i];
+        P.vel[0][i] = P.vel[1][i];
+    }
+    if (d->sela) {
+        P.tag[0][i] = P.tag[1][i];
+        P.mask[0][i] = P.mask[1][i];
+        P.origin[0][i] = P.origin[1][i];
+        P.image[0][i] = P.image[1][i];
     }
 }
 
@@ -442,10 +506,18 @@ hipError_t launch_first(const PoolDev &P, bool with_rho, int split, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_commit_step(const PoolDev &P, bool advance, hipStream_t st)
+hipError_t launch_commit_step(const PoolDev &P, int kind, int split, hipStream_t st)
 {
     if (P.nsys == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_commit_step, dim3(P.nsys), dim3(256), 0, st, P, advance ? 1 : 0);
+    hipLaunchKernelGGL(k_commit_step, dim3(P.nsys), dim3(256), 0, st, P, kind, split);
+    return hipGetLastError();
+}
+
+hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, hipStream_t st)
+{
+    if (npad <= 0) return hipSuccess;
+    if (s < 0 || s >= P.nsys || part_base < 0 || part_base + npad > P.cap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_harvest, dim3((npad + 255) / 256), dim3(256), 0, st, P, s, part_base, npad);
     return hipGetLastError();
 }
 
@@ -458,21 +530,21 @@ hipError_t launch_initial(const PoolDev &P, hipStream_t st)
 }
 
 template <int STYLE, int MODE, int SPLIT>
-static void launch_force_t(const PoolDev &P, bool uni, int nt, hipStream_t st)
+static void launch_force_t(const PoolDev &P, bool uni, int t0, int nt, hipStream_t st)
 {
     if (uni)
-        hipLaunchKernelGGL((k_force<STYLE, MODE, true, SPLIT>), dim3(xcd_grid(nt)), dim3(WAVE * SPLIT), 0, st, P, nt);
+        hipLaunchKernelGGL((k_force<STYLE, MODE, true, SPLIT>), dim3(xcd_grid(nt)), dim3(WAVE * SPLIT), 0, st, P, t0, nt);
     else
-        hipLaunchKernelGGL((k_force<STYLE, MODE, false, SPLIT>), dim3(xcd_grid(nt)), dim3(WAVE * SPLIT), 0, st, P, nt);
+        hipLaunchKernelGGL((k_force<STYLE, MODE, false, SPLIT>), dim3(xcd_grid(nt)), dim3(WAVE * SPLIT), 0, st, P, t0, nt);
 }
 
 template <int STYLE, int MODE>
-static hipError_t launch_force_s(const PoolDev &P, bool uni, int split, int nt, hipStream_t st)
+static hipError_t launch_force_s(const PoolDev &P, bool uni, int split, int t0, int nt, hipStream_t st)
 {
     switch (split) {
-    case 1: launch_force_t<STYLE, MODE, 1>(P, uni, nt, st); break;
-    case 2: launch_force_t<STYLE, MODE, 2>(P, uni, nt, st); break;
-    case 4: launch_force_t<STYLE, MODE, 4>(P, uni, nt, st); break;
+    case 1: launch_force_t<STYLE, MODE, 1>(P, uni, t0, nt, st); break;
+    case 2: launch_force_t<STYLE, MODE, 2>(P, uni, t0, nt, st); break;
+    case 4: launch_force_t<STYLE, MODE, 4>(P, uni, t0, nt, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -480,18 +552,28 @@ static hipError_t launch_force_s(const PoolDev &P, bool uni, int split, int nt, 
 
 hipError_t launch_force(const PoolDev &P, int style, int mode, bool uni, int split, hipStream_t st)
 {
-    const int nt = P.cap / WAVE;
-    if (nt == 0) return hipSuccess;
+    return launch_force_range(P, style, mode, uni, split, 0, P.cap / WAVE, st);
+}
+
+hipError_t launch_force_range(const PoolDev &P, int style, int mode, bool uni, int split, int t0, int nt,
+                              hipStream_t st)
+{
+    if (nt <= 0) return hipSuccess;
+    if (t0 < 0 || t0 + nt > P.cap / WAVE) return hipErrorInvalidValue;
 #define MDPD_FORCE_CASE(S, M) \
-    if (style == S && mode == M) return launch_force_s<S, M>(P, uni, split, nt, st);
+    if (style == S && mode == M) return launch_force_s<S, M>(P, uni, split, t0, nt, st);
     MDPD_FORCE_CASE(STYLE_MDPD, MODE_SETUP)
     MDPD_FORCE_CASE(STYLE_MDPD, MODE_FUSED)
     MDPD_FORCE_CASE(STYLE_MDPD, MODE_FUSED_OUT)
     MDPD_FORCE_CASE(STYLE_MDPD, MODE_FINAL)
+    MDPD_FORCE_CASE(STYLE_MDPD, MODE_PHASED)
+    MDPD_FORCE_CASE(STYLE_MDPD, MODE_HARVEST)
     MDPD_FORCE_CASE(STYLE_DPD, MODE_SETUP)
     MDPD_FORCE_CASE(STYLE_DPD, MODE_FUSED)
     MDPD_FORCE_CASE(STYLE_DPD, MODE_FUSED_OUT)
     MDPD_FORCE_CASE(STYLE_DPD, MODE_FINAL)
+    MDPD_FORCE_CASE(STYLE_DPD, MODE_PHASED)
+    MDPD_FORCE_CASE(STYLE_DPD, MODE_HARVEST)
 #undef MDPD_FORCE_CASE
     return hipErrorInvalidValue;
 }

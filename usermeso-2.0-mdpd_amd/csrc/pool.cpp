@@ -1,0 +1,1002 @@
+// Pooled engine: many independent periodic systems ("jobs") share one set of
+// device arrays and advance together, one launch sequence per pass for the
+// whole pool.  It serves both the batched mode (submit many, run to the end)
+// and the served mode (jobs arrive while others run, are admitted as particle
+// slots free up, and leave when done).
+//
+// Reference counterparts: the ensemble mode of the reference runs independent
+// cases as separate MPI communicators, one process (and launch stream) per
+// case (src/main.cpp:49-117); here they are packed into one launch per pass.
+// Each job follows the same per-step algorithm as a plain run
+// (ModifiedVerlet::run, src/USER-MESO/mvv_meso.cu:309-572), including the
+// reneighbor decision of Neighbor::decide (src/neighbor.cpp:1216-1290), made
+// per system on the device.
+//
+// Device side of one pass (all systems at once, every launch skips systems it
+// has nothing to do for):
+//   re-sort + neighbor rows of the stale systems, commit,
+//   density / compact rows, force with the fused velocity-Verlet epilogue
+//   (MODE_PHASED: each system's own lifecycle), step commit + decide.
+// A pass is captured once in a hipGraph and replayed.  The host reads a
+// per-system status word every few passes (asynchronously, one batch behind):
+// finished systems are harvested (energy/virial pass + copy-out), their
+// particle and cell ranges freed, and queued jobs admitted into the space.
+#include "system.hpp"
+
+#include <chrono>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+#include <set>
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// first fit over [0, cap) with coalescing of freed ranges
+struct RangeAlloc {
+    std::map<long long, long long> fr;   // start -> length
+    long long cap = 0, used = 0;
+    void reset(long long c)
+    {
+        fr.clear();
+        cap = c;
+        used = 0;
+        if (c > 0) fr[0] = c;
+    }
+    long long take(long long len)
+    {
+        for (auto it = fr.begin(); it != fr.end(); ++it) {
+            if (it->second < len) continue;
+            const long long st = it->first, rest = it->second - len;
+            fr.erase(it);
+            if (rest > 0) fr[st + len] = rest;
+            used += len;
+            return st;
+        }
+        return -1;
+    }
+    void give(long long st, long long len)
+    {
+        used -= len;
+        auto it = fr.emplace(st, len).first;
+        auto nx = std::next(it);
+        if (nx != fr.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            fr.erase(nx);
+        }
+        if (it != fr.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                fr.erase(it);
+            }
+        }
+    }
+};
+
+// page-locked staging buffers, reused by size (asynchronous copies need them)
+struct PinnedCache {
+    std::multimap<size_t, void *> free_;
+    ~PinnedCache()
+    {
+        for (auto &f : free_) (void)hipHostFree(f.second);
+    }
+    void *get(size_t bytes, size_t &got)
+    {
+        auto it = free_.lower_bound(bytes);
+        if (it != free_.end()) {
+            got = it->first;
+            void *p = it->second;
+            free_.erase(it);
+            return p;
+        }
+        size_t sz = (size_t)1 << 20;
+        while (sz < bytes) sz <<= 1;
+        void *p = nullptr;
+        hip_check(hipHostMalloc(&p, sz, hipHostMallocDefault), "hipHostMalloc");
+        got = sz;
+        return p;
+    }
+    void put(void *p, size_t sz) { free_.emplace(sz, p); }
+};
+
+enum JobState { JOB_QUEUED = 0, JOB_RUNNING = 1, JOB_DONE = 2 };
+
+struct Job {
+    int id = 0, state = JOB_QUEUED;
+    mdpd_system::Desc ds;
+    int n = 0, npad = 0, ncell = 0, ntypes = 0;
+    long long nsteps = 0;
+    double volume = 0;
+    double mass[MAXTYPES + 1] = {0};
+    std::vector<float4> hp, hv;           // packed input, npad slots
+    std::vector<int> ht, hm, ho, himg;
+    // placement while admitted
+    int slot = -1;
+    long long part_base = -1, cell_base = -1;
+    // harvest in flight
+    bool harvesting = false;
+    void *stage = nullptr;
+    size_t stage_bytes = 0;
+    hipEvent_t ev = nullptr;
+    // results, creation order
+    std::vector<double> x, v, f, e, vir, rho;
+    std::vector<int> tag, type, mask, image;
+    long long step_end = 0;
+    int nbuild = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+// bytes of device memory per particle slot (state, scratch, 2 cell slots)
+// and per neighbor-row entry (nlist + clist)
+constexpr double BYTES_PER_SLOT = 256.0;
+constexpr double BYTES_PER_ENTRY = 8.0;
+constexpr int CELLS_PER_SLOT = 2;
+
+}  // namespace
+
+struct mdpd_pool {
+    int device = 0;
+    std::string err;
+    hipStream_t stream = nullptr;
+    double budget = 0;
+    int max_sys = 0;
+
+    // ---- device arrays (allocated at the first admission) ----
+    bool allocated = false;
+    int cap = 0, nslots = 0, maxnbr = 0;
+    DevBuf<float4> pos[2], vel[2], xhold, frc, vend;
+    DevBuf<int> tag[2], mask[2], origin[2], image[2];
+    DevBuf<float> rho, vir, coef, minv, mmass;
+    DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys;
+    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl, wl, work_total;
+    DevBuf<SysDev> dsys;
+    RangeAlloc parts, cells;              // parts in units of 256 slots
+    std::vector<int> free_slots, slot_job;
+
+    // ---- jobs ----
+    std::map<int, std::unique_ptr<Job>> jobs;
+    std::deque<int> queue;
+    std::vector<int> finished;            // done since the last mdpd_pool_done call
+    int next_id = 1, nrunning = 0, nharvest = 0;
+
+    // ---- launch shape ----
+    int split = 2;
+    bool autotune = true, tuned = false;
+    double tune_ms[5] = {0, 0, 0, 0, 0};
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_valid = false;
+    int passes_per_poll = 8;
+    std::set<std::pair<int, int>> combos;  // (style, single type) of admitted jobs
+    bool with_rho = false;
+
+    // ---- status polling and staging ----
+    int *pin_status[2] = {nullptr, nullptr};
+    hipEvent_t ev_poll[2] = {nullptr, nullptr};
+    bool poll_pending[2] = {false, false};
+    int poll_cur = 0;
+    bool need_regrow = false;
+    PinnedCache pinned;
+    std::vector<std::pair<hipEvent_t, std::pair<void *, size_t>>> uploads;
+    std::vector<hipEvent_t> ev_free;
+
+    // ---- counters ----
+    long long passes = 0, regrows = 0;
+    double particle_steps = 0;
+
+    ~mdpd_pool()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto &u : uploads) {
+            (void)hipEventDestroy(u.first);
+            (void)hipHostFree(u.second.first);
+        }
+        for (auto &j : jobs) {
+            if (j.second->ev) (void)hipEventDestroy(j.second->ev);
+            if (j.second->stage) (void)hipHostFree(j.second->stage);
+        }
+        for (auto e : ev_free) (void)hipEventDestroy(e);
+        for (int k = 0; k < 2; ++k) {
+            if (ev_poll[k]) (void)hipEventDestroy(ev_poll[k]);
+            if (pin_status[k]) (void)hipHostFree(pin_status[k]);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    hipEvent_t take_event()
+    {
+        if (!ev_free.empty()) {
+            hipEvent_t e = ev_free.back();
+            ev_free.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        return e;
+    }
+
+    void ensure_stream()
+    {
+        if (stream) return;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || device >= ndev)
+            fail(MDPD_E_HIP, "HIP device %d is not available", device);
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (int k = 0; k < 2; ++k) {
+            hip_check(hipHostMalloc((void **)&pin_status[k], sizeof(int) * max_sys, hipHostMallocDefault),
+                      "hipHostMalloc");
+            hip_check(hipEventCreateWithFlags(&ev_poll[k], hipEventDisableTiming), "hipEventCreate");
+        }
+    }
+
+    PoolDev view()
+    {
+        PoolDev P{};
+        for (int b = 0; b < 2; ++b) {
+            P.pos[b] = pos[b].p; P.vel[b] = vel[b].p;
+            P.tag[b] = tag[b].p; P.mask[b] = mask[b].p; P.origin[b] = origin[b].p; P.image[b] = image[b].p;
+        }
+        P.xhold = xhold.p; P.rho = rho.p; P.frc = frc.p; P.vir = vir.p; P.vend = vend.p;
+        P.pcell = pcell.p; P.pslot = pslot.p; P.psub = psub.p; P.tmp = tmp.p; P.key = nullptr;
+        P.cell_count = cell_count.p; P.cell_start = cell_start.p; P.cell_xyz = cell_xyz.p;
+        P.cell_rank = cell_rank.p; P.cell_sys = cell_sys.p;
+        P.nlist = nlist.p; P.ncount = ncount.p; P.wave_sys = wave_sys.p;
+        P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
+        P.cl = cl.p; P.wl = wl.p; P.work_total = work_total.p;
+        P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
+        P.cap = cap; P.ncellslots = nslots; P.nsys = max_sys; P.maxnbr = maxnbr;
+        return P;
+    }
+
+    double fixed_bytes() const
+    {
+        return (double)max_sys * (sizeof(SysDev) + sizeof(float) * (MAXTYPES * MAXTYPES * NCOEF + 2 * MAXTYPES) + 4);
+    }
+    // particle slots a budget holds at a given row width
+    long long capacity_for(int mnb) const
+    {
+        const double per = BYTES_PER_SLOT + BYTES_PER_ENTRY * mnb;
+        const double c = (budget - fixed_bytes()) / per;
+        return c < 256 ? 0 : (long long)(c / 256) * 256;
+    }
+    double device_bytes() const
+    {
+        return fixed_bytes() + (BYTES_PER_SLOT + BYTES_PER_ENTRY * maxnbr) * (double)cap;
+    }
+
+    // ------------------------------------------------------------------
+    // submit: copy a configured system's state and atoms
+    // ------------------------------------------------------------------
+    int submit(mdpd_system *sys, long long nsteps)
+    {
+        if (!sys) fail(MDPD_E_ARG, "null system");
+        if (nsteps < 1) fail(MDPD_E_ARG, "a job needs at least one step");
+        if (!sys->run_style_ok) fail(MDPD_E_STATE, "run_style mvv/meso must be set for the USER-MESO styles");
+        if (sys->setup_done) sys->pull_state();
+        auto j = std::make_unique<Job>();
+        sys->describe(j->ds);
+        j->n = j->ds.d.n;
+        j->npad = j->ds.d.npad;
+        j->ncell = j->ds.d.ncell;
+        j->ntypes = sys->ntypes;
+        j->nsteps = nsteps;
+        j->volume = (sys->hi[0] - sys->lo[0]) * (sys->hi[1] - sys->lo[1]) * (sys->hi[2] - sys->lo[2]);
+        for (int t = 0; t <= MAXTYPES; ++t) j->mass[t] = sys->mass[t];
+        const int mnb = std::max(maxnbr, j->ds.want_maxnbr);
+        const long long c = allocated ? cap : capacity_for(mnb);
+        const long long cs = allocated ? nslots : c * CELLS_PER_SLOT;
+        if (j->npad > c || j->ncell + 1 > cs)
+            fail(MDPD_E_NOMEM, "job of %d atoms / %d cells does not fit a pool of %lld slots / %lld cells", j->n,
+                 j->ncell, c, cs);
+        j->hp.resize(j->npad); j->hv.resize(j->npad);
+        j->ht.resize(j->npad); j->hm.resize(j->npad); j->ho.resize(j->npad); j->himg.resize(j->npad);
+        sys->pack(j->npad, j->hp.data(), j->hv.data(), j->ht.data(), j->hm.data(), j->ho.data(), j->himg.data());
+        j->id = next_id++;
+        if (next_id >= (1 << 27)) next_id = 1;
+        j->t_submit = now_s();
+        const int id = j->id;
+        jobs[id] = std::move(j);
+        queue.push_back(id);
+        return id;
+    }
+
+    // ------------------------------------------------------------------
+    // device arrays
+    // ------------------------------------------------------------------
+    // row width of the first allocation: the widest any queued job asks for
+    int planned_maxnbr()
+    {
+        int mnb = 32;
+        for (int id : queue) mnb = std::max(mnb, jobs[id]->ds.want_maxnbr);
+        return mnb;
+    }
+
+    void allocate()
+    {
+        ensure_stream();
+        maxnbr = planned_maxnbr();
+        cap = (int)std::min<long long>(capacity_for(maxnbr), (1LL << IMG_SHIFT) - 256);
+        if (cap < 256) fail(MDPD_E_NOMEM, "HBM budget %.3g bytes holds no particles", budget);
+        nslots = cap * CELLS_PER_SLOT;
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        for (int b = 0; b < 2; ++b) {
+            pos[b].alloc(cap); vel[b].alloc(cap);
+            tag[b].alloc(cap); mask[b].alloc(cap); origin[b].alloc(cap); image[b].alloc(cap);
+        }
+        xhold.alloc(cap); frc.alloc(cap); vend.alloc(cap);
+        rho.alloc(cap); vir.alloc((size_t)6 * cap);
+        pcell.alloc(cap); pslot.alloc(cap); psub.alloc(cap); tmp.alloc(cap);
+        ncount.alloc(cap); ccount.alloc((size_t)4 * cap);
+        nlist.alloc((size_t)cap * maxnbr);
+        clist.alloc((size_t)cap * maxnbr);
+        wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE); status.alloc(max_sys);
+        cl.alloc(cap); wl.alloc(cap); work_total.alloc(1);
+        cell_count.alloc(nslots); cell_start.alloc(nslots); cell_xyz.alloc(nslots);
+        cell_rank.alloc(nslots); cell_sys.alloc(nslots);
+        coef.alloc((size_t)max_sys * MAXTYPES * MAXTYPES * NCOEF);
+        minv.alloc((size_t)max_sys * MAXTYPES); mmass.alloc((size_t)max_sys * MAXTYPES);
+        dsys.alloc(max_sys);
+        hip_check(hipMemsetAsync(cell_count.p, 0, sizeof(int) * nslots, stream), "memset");
+        hip_check(hipMemsetAsync(cell_sys.p, 0xFF, sizeof(int) * nslots, stream), "memset");
+        hip_check(hipMemsetAsync(wave_sys.p, 0xFF, sizeof(int) * (cap / WAVE), stream), "memset");
+        hip_check(hipMemsetAsync(work_total.p, 0, sizeof(int), stream), "memset");
+        hip_check(hipMemsetAsync(status.p, 0, sizeof(int) * max_sys, stream), "memset");
+        hip_check(hipMemsetAsync(dsys.p, 0, sizeof(SysDev) * max_sys, stream), "memset");
+        hip_check(hipMemsetAsync(ncount.p, 0, sizeof(int) * cap, stream), "memset");
+        hip_check(hipStreamSynchronize(stream), "allocate");
+        parts.reset(cap / 256);
+        cells.reset(nslots);
+        free_slots.clear();
+        for (int s = max_sys - 1; s >= 0; --s) free_slots.push_back(s);
+        slot_job.assign(max_sys, 0);
+        allocated = true;
+        graph_valid = false;
+    }
+
+    // wider neighbor rows for every system, rows rebuilt in place (no re-sort,
+    // so results do not change); the stream is drained first
+    void regrow(int want)
+    {
+        hip_check(hipStreamSynchronize(stream), "regrow");
+        // status words read before the regrow are stale (overflow flags are
+        // cleared below; finished systems keep their word and show up again)
+        poll_pending[0] = poll_pending[1] = false;
+        for (int attempt = 0;; ++attempt) {
+            if (attempt > 6) fail(MDPD_E_OVERFLOW, "<MESO> Pair table overflow; local density too high");
+            maxnbr = std::max(want, (int)std::ceil(maxnbr * 1.5 / 16.0) * 16);
+            want = 0;
+            nlist.alloc((size_t)cap * maxnbr);
+            clist.alloc((size_t)cap * maxnbr);
+            graph_valid = false;
+            ++regrows;
+            PoolDev P = view();
+            hip_check(hipMemsetAsync(work_total.p, 0, sizeof(int), stream), "memset");
+            hip_check(launch_relist(P, stream), "relist");
+            hip_check(hipMemsetAsync(work_total.p, 0, sizeof(int), stream), "memset");
+            std::vector<SysDev> h(max_sys);
+            hip_check(hipMemcpyAsync(h.data(), dsys.p, sizeof(SysDev) * max_sys, hipMemcpyDeviceToHost, stream),
+                      "readback");
+            hip_check(hipStreamSynchronize(stream), "relist");
+            bool any = false;
+            for (int s = 0; s < max_sys; ++s) any |= slot_job[s] && h[s].active && h[s].overflow;
+            if (!any) break;
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // admission: FIFO, a job waits until its particle and cell ranges fit
+    // ------------------------------------------------------------------
+    bool admit()
+    {
+        bool any = false;
+        if (!queue.empty() && !allocated) allocate();
+        while (!queue.empty() && !free_slots.empty()) {
+            Job &j = *jobs[queue.front()];
+            const long long pu = j.npad / 256;
+            const long long pb = parts.take(pu);
+            if (pb < 0) break;
+            const long long cb = cells.take(j.ncell + 1);
+            if (cb < 0) {
+                parts.give(pb, pu);
+                break;
+            }
+            if (j.ds.want_maxnbr > maxnbr) regrow(j.ds.want_maxnbr);
+            const int s = free_slots.back();
+            free_slots.pop_back();
+            queue.pop_front();
+            upload(j, s, pb * 256, cb);
+            any = true;
+        }
+        return any;
+    }
+
+    void upload(Job &j, int s, long long pbase, long long cbase)
+    {
+        SysDev d = j.ds.d;
+        d.part_base = (int)pbase;
+        d.cell_base = (int)cbase;
+        d.coef_base = s * MAXTYPES * MAXTYPES * NCOEF;
+        d.mass_base = s * MAXTYPES;
+        d.maxnbr = maxnbr;
+        d.phase = PH_FRESH;
+        d.steps_left = (int)std::min<long long>(j.nsteps, 0x7FFFFFFF);
+        d.stale = 1;
+        d.active = 1;
+        d.overflow = 0;
+        d.ago = 0;
+        d.job = j.id;
+        d.sel = 0;
+        d.sela = 0;
+        d.nne = 0;
+        d.split_last = split;
+        const int npad = j.npad, nc1 = j.ncell + 1, ntw = npad / WAVE;
+        const size_t ncoef = j.ds.coef.size(), nt = j.ntypes;
+        // staging layout (every piece 16-byte aligned)
+        auto al = [](size_t b) { return (b + 15) / 16 * 16; };
+        size_t off[13], at = 0;
+        const size_t sz[13] = {sizeof(float4) * npad, sizeof(float4) * npad, sizeof(int) * npad, sizeof(int) * npad,
+                               sizeof(int) * npad, sizeof(int) * npad, sizeof(int) * nc1, sizeof(int) * nc1,
+                               sizeof(int) * nc1, sizeof(int) * ntw, sizeof(float) * ncoef, sizeof(float) * 2 * nt,
+                               sizeof(SysDev)};
+        for (int k = 0; k < 13; ++k) { off[k] = at; at += al(sz[k]); }
+        size_t got = 0;
+        char *st = (char *)pinned.get(at, got);
+        memcpy(st + off[0], j.hp.data(), sz[0]);
+        memcpy(st + off[1], j.hv.data(), sz[1]);
+        memcpy(st + off[2], j.ht.data(), sz[2]);
+        memcpy(st + off[3], j.hm.data(), sz[3]);
+        memcpy(st + off[4], j.ho.data(), sz[4]);
+        memcpy(st + off[5], j.himg.data(), sz[5]);
+        memcpy(st + off[6], j.ds.rank.data(), sz[6]);
+        memcpy(st + off[7], j.ds.xyz.data(), sz[7]);
+        int *cs = (int *)(st + off[8]);
+        for (int k = 0; k < nc1; ++k) cs[k] = s;
+        int *ws = (int *)(st + off[9]);
+        for (int k = 0; k < ntw; ++k) ws[k] = s;
+        memcpy(st + off[10], j.ds.coef.data(), sz[10]);
+        float *ms = (float *)(st + off[11]);
+        for (size_t t = 0; t < nt; ++t) { ms[t] = j.ds.mass[t]; ms[nt + t] = j.ds.minv[t]; }
+        memcpy(st + off[12], &d, sizeof(SysDev));
+        auto up = [&](void *dst, int k) {
+            hip_check(hipMemcpyAsync(dst, st + off[k], sz[k], hipMemcpyHostToDevice, stream), "upload");
+        };
+        up(pos[0].p + pbase, 0);
+        up(vel[0].p + pbase, 1);
+        up(tag[0].p + pbase, 2);
+        up(mask[0].p + pbase, 3);
+        up(origin[0].p + pbase, 4);
+        up(image[0].p + pbase, 5);
+        up(cell_rank.p + cbase, 6);
+        up(cell_xyz.p + cbase, 7);
+        up(cell_sys.p + cbase, 8);
+        up(wave_sys.p + pbase / WAVE, 9);
+        up(coef.p + d.coef_base, 10);
+        hip_check(hipMemcpyAsync(mmass.p + d.mass_base, ms, sizeof(float) * nt, hipMemcpyHostToDevice, stream), "upload");
+        hip_check(hipMemcpyAsync(minv.p + d.mass_base, ms + nt, sizeof(float) * nt, hipMemcpyHostToDevice, stream),
+                  "upload");
+        hip_check(hipMemsetAsync(cell_count.p + cbase, 0, sizeof(int) * nc1, stream), "memset");
+        up(dsys.p + s, 12);
+        hipEvent_t e = take_event();
+        hip_check(hipEventRecord(e, stream), "hipEventRecord");
+        uploads.push_back({e, {st, got}});
+
+        j.slot = s;
+        j.part_base = pbase;
+        j.cell_base = cbase;
+        j.state = JOB_RUNNING;
+        j.t_admit = now_s();
+        slot_job[s] = j.id;
+        ++nrunning;
+        const auto combo = std::make_pair(d.style, j.ntypes == 1 ? 1 : 0);
+        if (!combos.count(combo)) { combos.insert(combo); graph_valid = false; }
+        if (d.style == STYLE_MDPD && d.rho_bit && !with_rho) { with_rho = true; graph_valid = false; }
+        // the host copy of the input is no longer needed
+        std::vector<float4>().swap(j.hp); std::vector<float4>().swap(j.hv);
+        std::vector<int>().swap(j.ht); std::vector<int>().swap(j.hm);
+        std::vector<int>().swap(j.ho); std::vector<int>().swap(j.himg);
+    }
+
+    void recycle_uploads(bool block)
+    {
+        for (size_t k = 0; k < uploads.size();) {
+            auto &u = uploads[k];
+            const hipError_t q = block ? hipEventSynchronize(u.first) : hipEventQuery(u.first);
+            if (q == hipSuccess) {
+                pinned.put(u.second.first, u.second.second);
+                ev_free.push_back(u.first);
+                uploads[k] = uploads.back();
+                uploads.pop_back();
+            } else if (q == hipErrorNotReady) {
+                ++k;
+            } else {
+                hip_check(q, "upload event");
+            }
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // passes
+    // ------------------------------------------------------------------
+    void launch_pass(int sp)
+    {
+        PoolDev P = view();
+        hip_check(launch_rebuild(P, stream), "rebuild");
+        hip_check(launch_commit_rebuild(P, stream), "commit rebuild");
+        hip_check(launch_first(P, with_rho, sp, stream), "first pass");
+        for (auto &c : combos) hip_check(launch_force(P, c.first, MODE_PHASED, c.second != 0, sp, stream), "force");
+        hip_check(launch_commit_step(P, 2, sp, stream), "commit step");
+    }
+
+    void capture()
+    {
+        if (gexec) {
+            hip_check(hipGraphExecDestroy(gexec), "hipGraphExecDestroy");
+            gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            launch_pass(split);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "hipGraphInstantiate");
+        graph_valid = true;
+    }
+
+    void run_passes(int m)
+    {
+        if (m <= 0) return;
+        if (use_graph) {
+            if (!graph_valid) capture();
+            for (int k = 0; k < m; ++k) hip_check(hipGraphLaunch(gexec, stream), "hipGraphLaunch");
+        } else {
+            for (int k = 0; k < m; ++k) launch_pass(split);
+        }
+        passes += m;
+    }
+
+    // time each split on a few real passes (the work is not repeated: every
+    // pass advances the pool); results differ between splits only by FP32
+    // summation order
+    void tune()
+    {
+        tuned = true;
+        if (!autotune) return;
+        const int cand[3] = {1, 2, 4};
+        double ms[3] = {0, 0, 0};
+        hipEvent_t a, b;
+        hip_check(hipEventCreate(&a), "hipEventCreate");
+        hip_check(hipEventCreate(&b), "hipEventCreate");
+        for (int rep = 0; rep < 2; ++rep)
+            for (int k = 0; k < 3; ++k) {
+                const int c = rep ? cand[2 - k] : cand[k];
+                hip_check(hipEventRecord(a, stream), "hipEventRecord");
+                for (int p = 0; p < 4; ++p) launch_pass(c);
+                hip_check(hipEventRecord(b, stream), "hipEventRecord");
+                hip_check(hipEventSynchronize(b), "tune");
+                float t = 0;
+                hip_check(hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
+                ms[rep ? 2 - k : k] += t;
+                passes += 4;
+            }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        int best = 0;
+        for (int k = 1; k < 3; ++k)
+            if (ms[k] < ms[best]) best = k;
+        for (int k = 0; k < 3; ++k) tune_ms[cand[k]] = ms[k] / 8.0;
+        if (split != cand[best]) graph_valid = false;
+        split = cand[best];
+    }
+
+    // ------------------------------------------------------------------
+    // status, harvest, completion
+    // ------------------------------------------------------------------
+    void poll()
+    {
+        const int c = poll_cur;
+        hip_check(hipMemcpyAsync(pin_status[c], status.p, sizeof(int) * max_sys, hipMemcpyDeviceToHost, stream),
+                  "status");
+        hip_check(hipEventRecord(ev_poll[c], stream), "hipEventRecord");
+        poll_pending[c] = true;
+        const int p = c ^ 1;
+        if (poll_pending[p]) {
+            hip_check(hipEventSynchronize(ev_poll[p]), "status");
+            poll_pending[p] = false;
+            handle(pin_status[p]);
+        }
+        poll_cur = p;
+    }
+
+    void drain_polls()
+    {
+        hip_check(hipStreamSynchronize(stream), "drain");
+        for (int k = 0; k < 2; ++k) {
+            const int p = poll_cur ^ 1 ^ k;   // older first
+            if (poll_pending[p]) {
+                poll_pending[p] = false;
+                handle(pin_status[p]);
+            }
+        }
+    }
+
+    void handle(const int *st)
+    {
+        for (int s = 0; s < max_sys; ++s) {
+            const int jid = slot_job[s];
+            if (!jid) continue;
+            const int w = st[s];
+            if ((w >> 3) != jid) continue;          // no commit yet for this occupant
+            Job &j = *jobs[jid];
+            if (j.harvesting) continue;
+            if (w & 2) start_harvest(j);
+            if (w & 4) need_regrow = true;
+        }
+        if (need_regrow) {
+            need_regrow = false;
+            regrow(0);
+        }
+    }
+
+    void start_harvest(Job &j)
+    {
+        PoolDev P = view();
+        const int t0 = (int)(j.part_base / WAVE), nt = j.npad / WAVE;
+        const int style = j.ds.d.style;
+        const bool uni = j.ntypes == 1;
+        // energy / virial of the last pass; the launch whose split matches it runs
+        for (int sp : {1, 2, 4}) hip_check(launch_force_range(P, style, MODE_HARVEST, uni, sp, t0, nt, stream), "harvest");
+        hip_check(launch_harvest(P, j.slot, (int)j.part_base, j.npad, stream), "harvest");
+        const size_t np = j.npad;
+        const size_t bytes = np * (3 * sizeof(float4) + 5 * sizeof(int) + 6 * sizeof(float)) + sizeof(SysDev);
+        j.stage = pinned.get(bytes, j.stage_bytes);
+        char *o = (char *)j.stage;
+        auto dn = [&](const void *src, size_t b) {
+            hip_check(hipMemcpyAsync(o, src, b, hipMemcpyDeviceToHost, stream), "harvest copy");
+            o += b;
+        };
+        const long long pb = j.part_base;
+        dn(pos[0].p + pb, np * sizeof(float4));
+        dn(vel[0].p + pb, np * sizeof(float4));
+        dn(frc.p + pb, np * sizeof(float4));
+        dn(tag[0].p + pb, np * sizeof(int));
+        dn(mask[0].p + pb, np * sizeof(int));
+        dn(origin[0].p + pb, np * sizeof(int));
+        dn(image[0].p + pb, np * sizeof(int));
+        dn(rho.p + pb, np * sizeof(float));
+        for (int c = 0; c < 6; ++c) dn(vir.p + (size_t)c * cap + pb, np * sizeof(float));
+        dn(dsys.p + j.slot, sizeof(SysDev));
+        j.ev = take_event();
+        hip_check(hipEventRecord(j.ev, stream), "hipEventRecord");
+        j.harvesting = true;
+        --nrunning;
+        ++nharvest;
+    }
+
+    // finished copies -> results in creation order; ranges and slot freed
+    int finish_harvests(bool block)
+    {
+        int done = 0;
+        for (auto &kv : jobs) {
+            Job &j = *kv.second;
+            if (!j.harvesting) continue;
+            const hipError_t q = block ? hipEventSynchronize(j.ev) : hipEventQuery(j.ev);
+            if (q == hipErrorNotReady) continue;
+            hip_check(q, "harvest event");
+            const size_t np = j.npad;
+            const char *o = (const char *)j.stage;
+            const float4 *hp = (const float4 *)o; o += np * sizeof(float4);
+            const float4 *hv = (const float4 *)o; o += np * sizeof(float4);
+            const float4 *hf = (const float4 *)o; o += np * sizeof(float4);
+            const int *ht = (const int *)o; o += np * sizeof(int);
+            const int *hm = (const int *)o; o += np * sizeof(int);
+            const int *ho = (const int *)o; o += np * sizeof(int);
+            const int *hi = (const int *)o; o += np * sizeof(int);
+            const float *hr = (const float *)o; o += np * sizeof(float);
+            const float *hw = (const float *)o; o += 6 * np * sizeof(float);
+            SysDev d;
+            memcpy(&d, o, sizeof(SysDev));
+            const int n = j.n;
+            j.x.assign(3 * n, 0); j.v.assign(3 * n, 0); j.f.assign(3 * n, 0);
+            j.e.assign(n, 0); j.rho.assign(n, 0); j.vir.assign(6 * n, 0);
+            j.tag.assign(n, 0); j.type.assign(n, 0); j.mask.assign(n, 0); j.image.assign(3 * n, 0);
+            for (int k = 0; k < n; ++k) {
+                const int i = ho[k];
+                if (i < 0 || i >= n) fail(MDPD_E_HIP, "harvest of job %d: bad particle origin %d", j.id, i);
+                j.x[3 * i] = hp[k].x; j.x[3 * i + 1] = hp[k].y; j.x[3 * i + 2] = hp[k].z;
+                j.v[3 * i] = hv[k].x; j.v[3 * i + 1] = hv[k].y; j.v[3 * i + 2] = hv[k].z;
+                j.f[3 * i] = hf[k].x; j.f[3 * i + 1] = hf[k].y; j.f[3 * i + 2] = hf[k].z;
+                j.e[i] = hf[k].w;
+                j.rho[i] = hr[k];
+                for (int c = 0; c < 6; ++c) j.vir[6 * i + c] = hw[(size_t)c * np + k];
+                j.tag[i] = ht[k];
+                int ty;
+                memcpy(&ty, &hp[k].w, 4);
+                j.type[i] = ty + 1;
+                j.mask[i] = hm[k];
+                j.image[3 * i] = (hi[k] & 1023) - 512;
+                j.image[3 * i + 1] = ((hi[k] >> 10) & 1023) - 512;
+                j.image[3 * i + 2] = ((hi[k] >> 20) & 1023) - 512;
+            }
+            j.step_end = d.step;
+            pinned.put(j.stage, j.stage_bytes);
+            j.stage = nullptr;
+            ev_free.push_back(j.ev);
+            j.ev = nullptr;
+            // free the slot and ranges; tiles unmapped before any new occupant
+            hip_check(hipMemsetAsync(wave_sys.p + j.part_base / WAVE, 0xFF, sizeof(int) * (j.npad / WAVE), stream),
+                      "memset");
+            hip_check(hipMemsetAsync(cell_sys.p + j.cell_base, 0xFF, sizeof(int) * (j.ncell + 1), stream), "memset");
+            parts.give(j.part_base / 256, j.npad / 256);
+            cells.give(j.cell_base, j.ncell + 1);
+            slot_job[j.slot] = 0;
+            free_slots.push_back(j.slot);
+            j.slot = -1;
+            j.harvesting = false;
+            j.state = JOB_DONE;
+            j.t_done = now_s();
+            --nharvest;
+            particle_steps += (double)j.n * (double)j.nsteps;
+            finished.push_back(j.id);
+            ++done;
+        }
+        return done;
+    }
+
+    // ------------------------------------------------------------------
+    // drive the device.  Returns after max_passes passes (0: no limit), when
+    // stop_on_done and a job finished, or when every job is done.
+    // ------------------------------------------------------------------
+    int advance(long long max_passes, bool stop_on_done)
+    {
+        int ndone = 0;
+        long long launched = 0;
+        if (queue.empty() && nrunning == 0 && nharvest == 0) return 0;
+        ensure_stream();
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        for (;;) {
+            ndone += finish_harvests(false);
+            recycle_uploads(false);
+            admit();
+            if (stop_on_done && ndone > 0) break;
+            if (nrunning == 0) {
+                if (nharvest > 0) {
+                    drain_polls();
+                    ndone += finish_harvests(true);
+                    continue;
+                }
+                if (poll_pending[0] || poll_pending[1]) {
+                    drain_polls();
+                    continue;
+                }
+                if (queue.empty()) break;
+                fail(MDPD_E_NOMEM, "job %d does not fit the free pool", queue.front());
+            }
+            if (max_passes > 0 && launched >= max_passes) break;
+            if (!tuned) {
+                tune();
+                launched += 24;
+                poll();
+                continue;
+            }
+            long long m = passes_per_poll;
+            if (max_passes > 0) m = std::min(m, max_passes - launched);
+            run_passes((int)m);
+            launched += m;
+            poll();
+        }
+        if (nrunning == 0 && nharvest == 0 && queue.empty()) {
+            drain_polls();
+            recycle_uploads(true);
+        }
+        return ndone;
+    }
+
+    Job &job(int id)
+    {
+        auto it = jobs.find(id);
+        if (it == jobs.end()) fail(MDPD_E_ARG, "unknown job %d", id);
+        return *it->second;
+    }
+};
+
+#define POOL_GUARD(pool, ...)                                  \
+    do {                                                       \
+        if (!(pool)) return MDPD_E_ARG;                        \
+        try {                                                  \
+            (pool)->err.clear();                               \
+            __VA_ARGS__;                                       \
+            return MDPD_OK;                                    \
+        } catch (const Fail &f) {                              \
+            (pool)->err = f.msg;                               \
+            return f.code;                                     \
+        } catch (const std::exception &e) {                    \
+            (pool)->err = e.what();                            \
+            return MDPD_E_ARG;                                 \
+        }                                                      \
+    } while (0)
+
+extern "C" {
+
+int mdpd_pool_open(int device, double hbm_budget_bytes, int max_systems, mdpd_pool **out)
+{
+    if (!out || device < 0 || max_systems < 1 || max_systems > 65536 || !(hbm_budget_bytes > 0)) return MDPD_E_ARG;
+    *out = nullptr;
+    mdpd_pool *p = new (std::nothrow) mdpd_pool();
+    if (!p) return MDPD_E_NOMEM;
+    p->device = device;
+    p->budget = hbm_budget_bytes;
+    p->max_sys = max_systems;
+    *out = p;
+    return MDPD_OK;
+}
+
+void mdpd_pool_close(mdpd_pool *pool)
+{
+    if (!pool) return;
+    (void)hipSetDevice(pool->device);
+    delete pool;
+}
+
+const char *mdpd_pool_last_error(const mdpd_pool *pool) { return pool ? pool->err.c_str() : "null handle"; }
+
+int mdpd_pool_submit(mdpd_pool *pool, mdpd_system *sys, long long nsteps, int *job)
+{
+    POOL_GUARD(pool, {
+        if (!job) fail(MDPD_E_ARG, "null job output");
+        *job = pool->submit(sys, nsteps);
+    });
+}
+
+int mdpd_pool_advance(mdpd_pool *pool, long long max_passes, int stop_on_done, int *ndone)
+{
+    POOL_GUARD(pool, {
+        const int d = pool->advance(max_passes, stop_on_done != 0);
+        if (ndone) *ndone = d;
+    });
+}
+
+int mdpd_pool_run(mdpd_pool *pool) { POOL_GUARD(pool, { pool->advance(0, false); }); }
+
+int mdpd_pool_job_state(mdpd_pool *pool, int job, int *state)
+{
+    POOL_GUARD(pool, {
+        if (!state) fail(MDPD_E_ARG, "null output");
+        *state = pool->job(job).state;
+    });
+}
+
+int mdpd_pool_done(mdpd_pool *pool, int *jobs, int max, int *count)
+{
+    POOL_GUARD(pool, {
+        if (!count || (max > 0 && !jobs)) fail(MDPD_E_ARG, "null output");
+        const int k = std::min<int>(max, (int)pool->finished.size());
+        for (int i = 0; i < k; ++i) jobs[i] = pool->finished[i];
+        pool->finished.erase(pool->finished.begin(), pool->finished.begin() + k);
+        *count = k;
+    });
+}
+
+int mdpd_pool_gather(mdpd_pool *pool, int job, const char *name, int type, int count, void *data)
+{
+    POOL_GUARD(pool, {
+        if (!name || !data) fail(MDPD_E_ARG, "null argument");
+        Job &j = pool->job(job);
+        if (j.state != JOB_DONE) fail(MDPD_E_STATE, "job %d has not finished", job);
+        const std::string nm(name);
+        auto want = [&](int t, int c) {
+            if (type != t || count != c) fail(MDPD_E_ARG, "gather of %s needs type %d count %d", name, t, c);
+        };
+        const size_t n = j.n;
+        if (nm == "x" || nm == "v" || nm == "f") {
+            want(1, 3);
+            const std::vector<double> &s = nm == "x" ? j.x : (nm == "v" ? j.v : j.f);
+            memcpy(data, s.data(), sizeof(double) * 3 * n);
+        } else if (nm == "e_pair" || nm == "rho") {
+            want(1, 1);
+            memcpy(data, (nm == "rho" ? j.rho : j.e).data(), sizeof(double) * n);
+        } else if (nm == "virial") {
+            want(1, 6);
+            memcpy(data, j.vir.data(), sizeof(double) * 6 * n);
+        } else if (nm == "tag" || nm == "type" || nm == "mask") {
+            want(0, 1);
+            memcpy(data, (nm == "tag" ? j.tag : (nm == "type" ? j.type : j.mask)).data(), sizeof(int) * n);
+        } else if (nm == "image") {
+            want(0, 3);
+            memcpy(data, j.image.data(), sizeof(int) * 3 * n);
+        } else {
+            fail(MDPD_E_ARG, "unknown per-atom quantity '%s'", name);
+        }
+    });
+}
+
+int mdpd_pool_thermo(mdpd_pool *pool, int job, double out[4])
+{
+    POOL_GUARD(pool, {
+        if (!out) fail(MDPD_E_ARG, "null output");
+        Job &j = pool->job(job);
+        if (j.state != JOB_DONE) fail(MDPD_E_STATE, "job %d has not finished", job);
+        thermo_of(j.n, j.mass, j.type.data(), j.v.data(), j.vir.data(), j.e.data(), j.volume, out);
+    });
+}
+
+int mdpd_pool_job_info(mdpd_pool *pool, int job, long long *natoms, long long *timestep, double times[3])
+{
+    POOL_GUARD(pool, {
+        Job &j = pool->job(job);
+        if (natoms) *natoms = j.n;
+        if (timestep) *timestep = j.step_end;
+        if (times) { times[0] = j.t_submit; times[1] = j.t_admit; times[2] = j.t_done; }
+    });
+}
+
+int mdpd_pool_release(mdpd_pool *pool, int job)
+{
+    POOL_GUARD(pool, {
+        Job &j = pool->job(job);
+        if (j.state != JOB_DONE) fail(MDPD_E_STATE, "job %d has not finished", job);
+        pool->jobs.erase(job);
+    });
+}
+
+int mdpd_pool_tune(mdpd_pool *pool, const char *knob, int value)
+{
+    POOL_GUARD(pool, {
+        if (!knob) fail(MDPD_E_ARG, "null knob");
+        const std::string k(knob);
+        if (k == "split") {
+            if (value != 0 && value != 1 && value != 2 && value != 4) fail(MDPD_E_ARG, "split must be 0 (auto), 1, 2 or 4");
+            pool->autotune = value == 0;
+            pool->tuned = value != 0;
+            if (value) pool->split = value;
+            pool->graph_valid = false;
+        } else if (k == "graph") {
+            pool->use_graph = value != 0;
+        } else if (k == "poll") {
+            if (value < 1 || value > 4096) fail(MDPD_E_ARG, "poll must be 1..4096 passes");
+            pool->passes_per_poll = value;
+        } else {
+            fail(MDPD_E_ARG, "unknown tuning knob '%s'", knob);
+        }
+    });
+}
+
+int mdpd_pool_stat(mdpd_pool *pool, const char *name, double *value)
+{
+    POOL_GUARD(pool, {
+        if (!name || !value) fail(MDPD_E_ARG, "null argument");
+        const std::string k(name);
+        if (k == "passes") *value = (double)pool->passes;
+        else if (k == "particle_steps") *value = pool->particle_steps;
+        else if (k == "split") *value = pool->split;
+        else if (k == "capacity") *value = pool->allocated ? pool->cap : (double)pool->capacity_for(pool->planned_maxnbr());
+        else if (k == "maxnbr") *value = pool->maxnbr;
+        else if (k == "bytes") *value = pool->allocated ? pool->device_bytes() : 0.0;
+        else if (k == "regrows") *value = (double)pool->regrows;
+        else if (k == "running") *value = pool->nrunning + pool->nharvest;
+        else if (k == "queued") *value = (double)pool->queue.size();
+        else if (k == "used_slots") *value = (double)pool->parts.used * 256.0;
+        else if (k == "tune_ms_1") *value = pool->tune_ms[1];
+        else if (k == "tune_ms_2") *value = pool->tune_ms[2];
+        else if (k == "tune_ms_4") *value = pool->tune_ms[4];
+        else fail(MDPD_E_ARG, "unknown statistic '%s'", name);
+    });
+}
+
+double mdpd_clock(void) { return now_s(); }
+
+}  // extern "C"

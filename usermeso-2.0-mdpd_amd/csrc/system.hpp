@@ -1,0 +1,958 @@
+// Host side of one mdpd355 system: the input-script surface of the mDPD/DPD
+// styles, device memory ownership and the plain (one system per launch) step
+// loop.  The C ABI over it is in engine.cpp; the pooled engine (pool.cpp)
+// reads a system's script state and atoms through describe() / pack().
+//
+// Reference counterparts:
+//   script commands    src/input.cpp:542-579, pair_mdpd_meso.cu:299-397,
+//                      pair_dpd_meso.cu:272-357, fix_nve_meso.cu:29-40,
+//                      fix_aboundary_rho_meso.cu:30-104, fix_addforce_meso.cu:38-47,
+//                      atom_vec_mdpd_atomic_meso.cu:19-54, src/force.cpp (bounds)
+//   setup / run        src/USER-MESO/mvv_meso.cu:203-283, 309-572
+//   bins               src/USER-MESO/neighbor_meso.cu:771-853
+//   gather / scatter   src/library.cpp (lammps_gather_atoms / lammps_scatter_atoms)
+#pragma once
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "../../include/mdpd355.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace mdpd {
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    throw Fail{code, buf};
+}
+
+inline void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) fail(MDPD_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            fail(MDPD_E_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+inline double parse_num(const std::string &s)
+{
+    char *end = nullptr;
+    double v = strtod(s.c_str(), &end);
+    if (!end || *end != '\0' || s.empty()) fail(MDPD_E_ARG, "expected a number, got '%s'", s.c_str());
+    return v;
+}
+
+inline long long parse_int(const std::string &s)
+{
+    char *end = nullptr;
+    long long v = strtoll(s.c_str(), &end, 10);
+    if (!end || *end != '\0' || s.empty()) fail(MDPD_E_ARG, "expected an integer, got '%s'", s.c_str());
+    return v;
+}
+
+// Force::bounds: "*", "n*", "*n", "m*n", "n" over 1..nmax
+inline void type_bounds(const std::string &s, int nmax, int &lo, int &hi)
+{
+    size_t star = s.find('*');
+    if (star == std::string::npos) {
+        lo = hi = (int)parse_int(s);
+    } else if (s.size() == 1) {
+        lo = 1; hi = nmax;
+    } else if (star == 0) {
+        lo = 1; hi = (int)parse_int(s.substr(1));
+    } else if (star == s.size() - 1) {
+        lo = (int)parse_int(s.substr(0, star)); hi = nmax;
+    } else {
+        lo = (int)parse_int(s.substr(0, star)); hi = (int)parse_int(s.substr(star + 1));
+    }
+    if (lo < 1 || hi > nmax || lo > hi) fail(MDPD_E_ARG, "numeric type range '%s' out of 1..%d", s.c_str(), nmax);
+}
+
+enum Phase { PH_REBUILD = 0, PH_DENSITY, PH_FORCE, PH_INITIAL, PH_COUNT };
+const char *const PHASE_NAMES[PH_COUNT] = {"rebuild", "density", "force", "initial"};
+
+struct FixDesc {
+    std::string id, style;
+    int groupbit = 0;
+    double rd = 0;       // aboundary/rho/meso
+    int lucy = 0;
+    double f[3] = {0, 0, 0};   // addforce/meso
+};
+
+// Thermo scalars of one system (compute_temp_meso.cu:49-101: dof = 3N - 3;
+// compute_pressure_meso.cu:115-155: P = (dof kT + sum of virial traces) / 3V):
+// out = {T, P, pe, ke}.  mass is 1-based by type.
+inline void thermo_of(int n, const double *mass, const int *type, const double *v, const double *vir,
+                      const double *e, double volume, double out[4])
+{
+    double ke = 0, vs = 0, pe = 0;
+    for (int i = 0; i < n; ++i) {
+        ke += mass[type[i]] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+        vs += vir[6 * i] + vir[6 * i + 1] + vir[6 * i + 2];
+        pe += e[i];
+    }
+    const double dof = 3.0 * n - 3.0;
+    const double T = dof > 0 ? ke / dof : 0;
+    out[0] = T;
+    out[1] = (dof * T + vs) / 3.0 / volume;
+    out[2] = pe;
+    out[3] = 0.5 * ke;
+}
+
+}  // namespace mdpd
+
+using namespace mdpd;
+
+struct mdpd_system {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // ---- script state ----
+    int atom_style = -1;              // STYLE_MDPD / STYLE_DPD
+    double rho_est = 0;
+    int pair_style = -1;
+    double temperature = 1.0, cut_global = 1.0;
+    int seed = 0;
+    int ntypes = 0;
+    bool box_set = false;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    double mass[MAXTYPES + 1] = {0};
+    bool mass_set[MAXTYPES + 1] = {false};
+    // pair coefficients (1-based, full matrix after init_one)
+    double cut[MAXTYPES + 1][MAXTYPES + 1] = {{0}}, cut_r[MAXTYPES + 1][MAXTYPES + 1] = {{0}};
+    double A[MAXTYPES + 1][MAXTYPES + 1] = {{0}}, B[MAXTYPES + 1][MAXTYPES + 1] = {{0}};
+    double gamma[MAXTYPES + 1][MAXTYPES + 1] = {{0}}, sigma[MAXTYPES + 1][MAXTYPES + 1] = {{0}};
+    double expw[MAXTYPES + 1][MAXTYPES + 1] = {{0}};
+    int setflag[MAXTYPES + 1][MAXTYPES + 1] = {{0}};
+    double dt = 0.005;                // LAMMPS lj default
+    double skin = 0.3;
+    int every = 1, delay = 10, check = 1;
+    bool run_style_ok = false;
+    std::vector<FixDesc> fixes;
+    std::vector<std::pair<std::string, int>> groups{{"all", 1}};
+    long long ntimestep = 0;
+    int thermo_every = 0;
+
+    // ---- atoms (creation order) ----
+    std::vector<int> h_tag, h_type, h_mask, h_image;
+    std::vector<double> h_x, h_v;
+
+    // ---- device state ----
+    bool setup_done = false;
+    bool dirty = true;                // configuration changed since setup
+    int cap = 0, maxnbr = 0, ncell = 0, nslots = 0;
+    long long nbuild = 0;
+    SysDev hsys{};
+    DevBuf<float4> pos[2], vel[2], xhold, frc, vend;
+    DevBuf<int> tag[2], mask[2], origin[2], image[2];
+    DevBuf<float> rho, vir, coef, minv, mmass;
+    DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys;
+    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl, wl, work_total;
+    DevBuf<unsigned long long> key;
+    DevBuf<SysDev> dsys;
+    int *pinned = nullptr;            // host-visible readback of flags
+
+    // ---- kernel timing with HIP events (phase -> total ms, launches) ----
+    bool prof = false;
+    int split = 2;                    // waves per 64-particle tile in the per-step passes
+    std::vector<hipEvent_t> ev_free;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open;
+    double prof_ms[PH_COUNT] = {0};
+    long long prof_n[PH_COUNT] = {0};
+
+    ~mdpd_system()
+    {
+        for (auto &e : ev_open) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
+        for (auto e : ev_free) (void)hipEventDestroy(e);
+        if (pinned) (void)hipHostFree(pinned);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // ------------------------------------------------------------------
+    int group_bit(const std::string &name) const
+    {
+        for (auto &g : groups)
+            if (g.first == name) return g.second;
+        return 0;
+    }
+    int need_group(const std::string &name) const
+    {
+        int b = group_bit(name);
+        if (!b) fail(MDPD_E_ARG, "could not find group ID '%s'", name.c_str());
+        return b;
+    }
+    const FixDesc *find_fix(const char *style) const
+    {
+        for (auto &f : fixes)
+            if (f.style == style) return &f;
+        return nullptr;
+    }
+
+    // ------------------------------------------------------------------
+    // input-script surface
+    // ------------------------------------------------------------------
+    void command(const std::string &raw)
+    {
+        std::string line = raw.substr(0, raw.find('#'));
+        std::istringstream is(line);
+        std::vector<std::string> w;
+        for (std::string t; is >> t;) w.push_back(t);
+        if (w.empty()) return;
+        const std::string &c = w[0];
+        auto need = [&](size_t n) {
+            if (w.size() < n) fail(MDPD_E_ARG, "illegal %s command", c.c_str());
+        };
+        if (c == "echo" || c == "log" || c == "print" || c == "newton" || c == "comm_modify" ||
+            c == "atom_modify" || c == "processors") {
+            return;
+        } else if (c == "units") {
+            need(2);
+            if (w[1] != "lj") fail(MDPD_E_UNSUPPORTED, "only units lj is supported");
+        } else if (c == "dimension") {
+            need(2);
+            if (w[1] != "3") fail(MDPD_E_UNSUPPORTED, "only dimension 3 is supported");
+        } else if (c == "boundary") {
+            need(4);
+            if (w[1] != "p" || w[2] != "p" || w[3] != "p") fail(MDPD_E_UNSUPPORTED, "only boundary p p p is supported");
+        } else if (c == "atom_style") {
+            need(2);
+            if (w[1] == "mdpd/atomic/meso") {
+                // atom_vec_mdpd_atomic_meso.cu:27-30: a density estimate is required
+                if (w.size() < 3)
+                    fail(MDPD_E_ARG, "For atom_style mdpd/atomic/meso, an estimation to maximum local particle density must be given. Or input 0 for default.");
+                atom_style = STYLE_MDPD;
+                rho_est = parse_num(w[2]);
+            } else if (w[1] == "dpd/atomic/meso") {
+                atom_style = STYLE_DPD;
+            } else {
+                fail(MDPD_E_UNSUPPORTED, "atom_style %s is outside this engine", w[1].c_str());
+            }
+            dirty = true;
+        } else if (c == "mass") {
+            need(3);
+            if (!box_set) fail(MDPD_E_STATE, "mass command before simulation box is defined");
+            int a, b;
+            type_bounds(w[1], ntypes, a, b);
+            double m = parse_num(w[2]);
+            if (m <= 0) fail(MDPD_E_ARG, "invalid mass value");
+            for (int t = a; t <= b; ++t) { mass[t] = m; mass_set[t] = true; }
+            dirty = true;
+        } else if (c == "group") {
+            cmd_group(w);
+        } else if (c == "pair_style") {
+            cmd_pair_style(w);
+        } else if (c == "pair_coeff") {
+            cmd_pair_coeff(w);
+        } else if (c == "fix") {
+            cmd_fix(w);
+        } else if (c == "unfix") {
+            need(2);
+            auto it = std::find_if(fixes.begin(), fixes.end(), [&](const FixDesc &f) { return f.id == w[1]; });
+            if (it == fixes.end()) fail(MDPD_E_ARG, "could not find fix ID '%s' to delete", w[1].c_str());
+            fixes.erase(it);
+            dirty = true;
+        } else if (c == "timestep") {
+            need(2);
+            dt = parse_num(w[1]);
+            if (dt <= 0) fail(MDPD_E_ARG, "illegal timestep");
+            dirty = true;
+        } else if (c == "neighbor") {
+            need(3);
+            skin = parse_num(w[1]);
+            if (skin < 0) fail(MDPD_E_ARG, "illegal neighbor command");
+            if (w[2] != "bin") fail(MDPD_E_UNSUPPORTED, "only neighbor style bin is supported");
+            dirty = true;
+        } else if (c == "neigh_modify") {
+            for (size_t k = 1; k < w.size(); k += 2) {
+                if (k + 1 >= w.size()) fail(MDPD_E_ARG, "illegal neigh_modify command");
+                if (w[k] == "every") every = (int)parse_int(w[k + 1]);
+                else if (w[k] == "delay") delay = (int)parse_int(w[k + 1]);
+                else if (w[k] == "check") check = w[k + 1] == "yes";
+                else fail(MDPD_E_UNSUPPORTED, "neigh_modify %s is not supported", w[k].c_str());
+            }
+            if (every < 1 || delay < 0) fail(MDPD_E_ARG, "illegal neigh_modify command");
+        } else if (c == "run_style") {
+            need(2);
+            if (w[1] != "mvv/meso") fail(MDPD_E_UNSUPPORTED, "run_style %s is outside this engine", w[1].c_str());
+            run_style_ok = true;
+        } else if (c == "reset_timestep") {
+            need(2);
+            ntimestep = parse_int(w[1]);
+            dirty = true;
+        } else if (c == "thermo") {
+            need(2);
+            thermo_every = (int)parse_int(w[1]);
+        } else if (c == "run") {
+            need(2);
+            run(parse_int(w[1]));
+        } else {
+            fail(MDPD_E_UNSUPPORTED, "unknown or unsupported command: %s", c.c_str());
+        }
+    }
+
+    void cmd_group(const std::vector<std::string> &w)
+    {
+        if (w.size() < 3) fail(MDPD_E_ARG, "illegal group command");
+        int bit = group_bit(w[1]);
+        if (!bit) {
+            if (groups.size() >= 32) fail(MDPD_E_ARG, "too many groups");
+            bit = 1 << groups.size();
+            groups.push_back({w[1], bit});
+        }
+        const size_t n = h_tag.size();
+        if (w[2] == "type" || w[2] == "id") {
+            for (size_t k = 3; k < w.size(); ++k) {
+                long long a, b;
+                size_t col = w[k].find(':');
+                if (col == std::string::npos) a = b = parse_int(w[k]);
+                else { a = parse_int(w[k].substr(0, col)); b = parse_int(w[k].substr(col + 1)); }
+                for (size_t i = 0; i < n; ++i) {
+                    long long v = w[2] == "type" ? h_type[i] : h_tag[i];
+                    if (v >= a && v <= b) h_mask[i] |= bit;
+                }
+            }
+        } else if (w[2] == "empty") {
+        } else {
+            fail(MDPD_E_UNSUPPORTED, "group style %s is not supported", w[2].c_str());
+        }
+        dirty = true;
+    }
+
+    void cmd_pair_style(const std::vector<std::string> &w)
+    {
+        if (w.size() < 2) fail(MDPD_E_ARG, "illegal pair_style command");
+        if (w[1] == "mdpd/meso") {
+            // MesoPairMDPD::settings, pair_mdpd_meso.cu:299-328
+            if (w.size() < 5)
+                fail(MDPD_E_ARG, "Illegal pair_style command:\n temperature cut_global seed fluid_group wall_group, or temperature cut_global seed ");
+            if (w.size() > 5) fail(MDPD_E_UNSUPPORTED, "the arbitrary-wall form of pair_style mdpd/meso is not built yet");
+            pair_style = STYLE_MDPD;
+            temperature = parse_num(w[2]);
+            cut_global = parse_num(w[3]);
+            seed = (int)parse_int(w[4]);
+        } else if (w[1] == "dpd/meso") {
+            // MesoPairDPD::settings, pair_dpd_meso.cu:272-287
+            if (w.size() != 4) fail(MDPD_E_ARG, "Illegal pair_style command");
+            pair_style = STYLE_DPD;
+            cut_global = parse_num(w[2]);
+            seed = (int)parse_int(w[3]);
+        } else {
+            fail(MDPD_E_UNSUPPORTED, "pair_style %s is outside this engine", w[1].c_str());
+        }
+        for (int i = 1; i <= MAXTYPES; ++i)
+            for (int j = i + 1; j <= MAXTYPES; ++j)
+                if (setflag[i][j]) cut[i][j] = cut_global;
+        dirty = true;
+    }
+
+    void cmd_pair_coeff(const std::vector<std::string> &w)
+    {
+        if (pair_style < 0) fail(MDPD_E_STATE, "pair_coeff command before pair_style is defined");
+        if (!box_set) fail(MDPD_E_STATE, "pair_coeff command before simulation box is defined");
+        std::vector<std::string> a(w.begin() + 1, w.end());
+        int ilo, ihi, jlo, jhi;
+        int count = 0;
+        if (pair_style == STYLE_MDPD) {
+            // MesoPairMDPD::coeff, pair_mdpd_meso.cu:330-365
+            if (a.size() != 7) fail(MDPD_E_ARG, "Incorrect args for mdpd pair coefficients: \n itype jtype A B gamma cutA cutB");
+            type_bounds(a[0], ntypes, ilo, ihi);
+            type_bounds(a[1], ntypes, jlo, jhi);
+            double a0 = parse_num(a[2]), b0 = parse_num(a[3]), g = parse_num(a[4]);
+            double c1 = parse_num(a[5]), c2 = parse_num(a[6]);
+            if (c1 < c2) fail(MDPD_E_ARG, "Incorrect args for pair coefficients:\n cutA should be larger than cutB.");
+            for (int i = ilo; i <= ihi; ++i)
+                for (int j = std::max(jlo, i); j <= jhi; ++j) {
+                    A[i][j] = a0; B[i][j] = b0; gamma[i][j] = g; cut[i][j] = c1; cut_r[i][j] = c2;
+                    setflag[i][j] = 1;
+                    ++count;
+                }
+        } else {
+            // MesoPairDPD::coeff, pair_dpd_meso.cu:289-323
+            if (a.size() < 6 || a.size() > 7) fail(MDPD_E_ARG, "Incorrect args for pair coefficients");
+            type_bounds(a[0], ntypes, ilo, ihi);
+            type_bounds(a[1], ntypes, jlo, jhi);
+            double a0 = parse_num(a[2]), g = parse_num(a[3]), s = parse_num(a[4]), e = parse_num(a[5]);
+            double c1 = a.size() == 7 ? parse_num(a[6]) : cut_global;
+            for (int i = ilo; i <= ihi; ++i)
+                for (int j = std::max(jlo, i); j <= jhi; ++j) {
+                    A[i][j] = a0; gamma[i][j] = g; sigma[i][j] = s; expw[i][j] = e; cut[i][j] = c1;
+                    setflag[i][j] = 1;
+                    ++count;
+                }
+        }
+        if (count == 0) fail(MDPD_E_ARG, "Incorrect args for pair coefficients");
+        dirty = true;
+    }
+
+    void cmd_fix(const std::vector<std::string> &w)
+    {
+        if (w.size() < 4) fail(MDPD_E_ARG, "illegal fix command");
+        FixDesc f;
+        f.id = w[1];
+        f.groupbit = need_group(w[2]);
+        f.style = w[3];
+        if (f.style == "nve/meso") {
+        } else if (f.style == "aboundary/rho/meso") {
+            // FixArbitraryBoundaryRho ctor, fix_aboundary_rho_meso.cu:46-104
+            for (size_t k = 4; k < w.size(); ++k) {
+                if (w[k] == "cut_rho" && k + 1 < w.size()) f.rd = parse_num(w[++k]);
+                else if (w[k] == "weight" && k + 1 < w.size()) {
+                    const std::string &m = w[++k];
+                    if (m == "lucy") f.lucy = 1;
+                    else if (m == "quadratic") f.lucy = 0;
+                    else fail(MDPD_E_ARG, "fix aboundary/rho/meso weight must be quadratic or lucy");
+                } else if (w[k] == "wall" || w[k] == "mobile" || w[k] == "cut_phi" || w[k] == "phi_c" || w[k] == "rho_wall") {
+                    fail(MDPD_E_UNSUPPORTED, "the arbitrary-boundary branch of fix aboundary/rho/meso is not built yet");
+                }
+            }
+            if (f.rd == 0) fail(MDPD_E_ARG, "<MESO> Fix aboundary/rho/meso parameters are not set properly.");
+        } else if (f.style == "addforce/meso") {
+            // MesoFixAddForce ctor, fix_addforce_meso.cu:38-47
+            if (w.size() < 7) fail(MDPD_E_ARG, "Illegal fix addforce/meso command");
+            for (int d = 0; d < 3; ++d) f.f[d] = parse_num(w[4 + d]);
+        } else {
+            fail(MDPD_E_UNSUPPORTED, "fix style %s is outside this engine", f.style.c_str());
+        }
+        auto it = std::find_if(fixes.begin(), fixes.end(), [&](const FixDesc &g) { return g.id == f.id; });
+        if (it != fixes.end()) *it = f;
+        else fixes.push_back(f);
+        dirty = true;
+    }
+
+    // ------------------------------------------------------------------
+    // setup: device tables and the first build + force pass
+    // ------------------------------------------------------------------
+    void init_coeffs(std::vector<float> &tab)
+    {
+        // init_one: pair_mdpd_meso.cu:382-397 (sigma = sqrt(2 kB T gamma)),
+        // pair_dpd_meso.cu:343-357 (sigma given)
+        for (int i = 1; i <= ntypes; ++i)
+            for (int j = i; j <= ntypes; ++j) {
+                if (!setflag[i][j]) fail(MDPD_E_STATE, "All pair coeffs are not set");
+                if (pair_style == STYLE_MDPD) sigma[i][j] = std::sqrt(2.0 * temperature * gamma[i][j]);
+                cut[j][i] = cut[i][j]; cut_r[j][i] = cut_r[i][j]; A[j][i] = A[i][j]; B[j][i] = B[i][j];
+                gamma[j][i] = gamma[i][j]; sigma[j][i] = sigma[i][j]; expw[j][i] = expw[i][j];
+            }
+        tab.assign((size_t)ntypes * ntypes * NCOEF, 0.f);
+        for (int i = 1; i <= ntypes; ++i)
+            for (int j = 1; j <= ntypes; ++j) {
+                float *c = &tab[((size_t)(i - 1) * ntypes + (j - 1)) * NCOEF];
+                c[C_RC] = (float)cut[i][j];
+                c[C_RCINV] = (float)(1.0 / cut[i][j]);
+                c[C_RD] = (float)cut_r[i][j];
+                c[C_RDINV] = cut_r[i][j] > 0 ? (float)(1.0 / cut_r[i][j]) : 0.f;
+                c[C_A] = (float)A[i][j];
+                c[C_B] = (float)B[i][j];
+                c[C_GAMMA] = (float)gamma[i][j];
+                c[C_SIGMA] = (float)sigma[i][j];
+                c[C_EXPW] = (float)expw[i][j];
+                c[C_RC2] = (float)(cut[i][j] * cut[i][j]);
+            }
+    }
+
+    double max_cut() const
+    {
+        double m = 0;
+        for (int i = 1; i <= ntypes; ++i)
+            for (int j = 1; j <= ntypes; ++j) m = std::max(m, cut[i][j]);
+        return m;
+    }
+
+    void ensure_stream()
+    {
+        if (stream) return;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || device >= ndev)
+            fail(MDPD_E_HIP, "HIP device %d is not available", device);
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipHostMalloc((void **)&pinned, 64 * sizeof(int), hipHostMallocDefault), "hipHostMalloc");
+    }
+
+    PoolDev view()
+    {
+        PoolDev P{};
+        for (int b = 0; b < 2; ++b) {
+            P.pos[b] = pos[b].p; P.vel[b] = vel[b].p;
+            P.tag[b] = tag[b].p; P.mask[b] = mask[b].p; P.origin[b] = origin[b].p; P.image[b] = image[b].p;
+        }
+        P.xhold = xhold.p; P.rho = rho.p; P.frc = frc.p; P.vir = vir.p; P.vend = vend.p;
+        P.pcell = pcell.p; P.pslot = pslot.p; P.psub = psub.p; P.tmp = tmp.p; P.key = key.p;
+        P.cell_count = cell_count.p; P.cell_start = cell_start.p; P.cell_xyz = cell_xyz.p;
+        P.cell_rank = cell_rank.p; P.cell_sys = cell_sys.p;
+        P.nlist = nlist.p; P.ncount = ncount.p; P.wave_sys = wave_sys.p;
+        P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
+        P.cl = cl.p; P.wl = wl.p; P.work_total = work_total.p;
+        P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
+        P.cap = cap; P.ncellslots = nslots; P.nsys = 1; P.maxnbr = maxnbr;
+        return P;
+    }
+
+    void validate()
+    {
+        if (!box_set) fail(MDPD_E_STATE, "run command before simulation box is defined");
+        if (atom_style < 0) fail(MDPD_E_STATE, "atom_style mdpd/atomic/meso or dpd/atomic/meso must be set");
+        if (pair_style < 0) fail(MDPD_E_STATE, "pair style must be set");
+        if (pair_style == STYLE_MDPD && atom_style != STYLE_MDPD)
+            fail(MDPD_E_STATE, "pair_style mdpd/meso requires atom_style mdpd/atomic/meso");
+        if (h_tag.empty()) fail(MDPD_E_STATE, "no atoms");
+        for (int t = 1; t <= ntypes; ++t)
+            if (!mass_set[t]) fail(MDPD_E_STATE, "Not all per-type masses are set");
+        if ((long long)h_tag.size() >= (1LL << IMG_SHIFT)) fail(MDPD_E_OVERFLOW, "too many atoms for one handle");
+    }
+
+    // Device-free description of this system for a launch: the per-system
+    // descriptor (bases left at 0), the FP32 coefficient table, per-type
+    // masses, the Morton rank of each interior cell and the packed coordinates
+    // of the cell at each rank, and the neighbor row width it asks for.
+    struct Desc {
+        SysDev d{};
+        std::vector<float> coef, mass, minv;
+        std::vector<int> rank, xyz;
+        int want_maxnbr = 0;
+    };
+
+    void describe(Desc &out)
+    {
+        validate();
+        init_coeffs(out.coef);
+        const int n = (int)h_tag.size();
+        const double rcut = max_cut();
+        const double cutneigh = rcut + skin;
+
+        // ---- bins (neighbor_meso.cu:837-853) ----
+        SysDev &d = out.d;
+        d = SysDev{};
+        for (int a = 0; a < 3; ++a) {
+            const double L = hi[a] - lo[a];
+            int nb = (int)(L * (1.0 / cutneigh));
+            if (nb < 1) nb = 1;
+            if (L < cutneigh) fail(MDPD_E_ARG, "Cannot use neighbor bins - box size << cutoff");
+            if (nb > 1000) fail(MDPD_E_OVERFLOW, "more than 1000 cells along one axis");
+            d.lo[a] = (float)lo[a];
+            d.hi[a] = (float)hi[a];
+            d.L[a] = (float)L;
+            d.lo_d[a] = lo[a];
+            d.binsize_d[a] = L / nb;
+            d.bininv_d[a] = 1.0 / d.binsize_d[a];
+            d.mbin[a] = nb + 2;
+            d.nb[a] = nb;
+        }
+        d.ncell = d.nb[0] * d.nb[1] * d.nb[2];
+        d.l1_shift = 12;
+        d.n = n;
+        d.npad = (n + 255) / 256 * 256;
+        d.rc2_list = (float)(cutneigh * cutneigh);
+        d.rc2_max = (float)(rcut * rcut);
+        d.trig2 = (float)(0.25 * skin * skin);
+        d.dt = (float)dt;
+        d.dtf = (float)(0.5 * dt);
+        d.dtinvsqrt = (float)(1.0 / std::sqrt(dt));
+        d.seed = (uint32_t)seed;
+        d.style = pair_style;
+        d.ntypes = ntypes;
+        const FixDesc *fn = find_fix("nve/meso"), *fr = find_fix("aboundary/rho/meso"), *fa = find_fix("addforce/meso");
+        d.nve_bit = fn ? fn->groupbit : 0;
+        d.rho_bit = fr ? fr->groupbit : 0;
+        d.add_bit = fa ? fa->groupbit : 0;
+        for (int a = 0; a < 3; ++a) d.add[a] = fa ? (float)fa->f[a] : 0.f;
+        if (fr) {
+            d.rd = (float)fr->rd;
+            d.lucy = fr->lucy;
+            const double r = fr->rd;
+            // fix_aboundary_rho_meso.cu:88-90
+            d.rho_factor = fr->lucy ? (float)(105.0 / (16.0 * M_PI * r * r * r * r * r * r * r))
+                                    : (float)(15.0 / (2.0 * M_PI * r * r * r * r * r));
+        }
+        d.stale = 1;
+        d.active = 1;
+        d.phase = PH_RUN;      // signatures come from launch_signatures, not the re-sort
+        d.step = ntimestep;
+        d.seed_cur = premix_host((uint32_t)seed, (uint32_t)ntimestep, 64);
+        d.seed_next = premix_host((uint32_t)seed, (uint32_t)(ntimestep + 1), 64);
+        d.n_every = every;
+        d.n_delay = delay;
+        d.n_check = check;
+
+        // ---- neighbor row width: 2x the expected count at rho_est ----
+        const double dens = rho_est > 0 ? rho_est : n / ((hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]));
+        const double expect = dens * 4.0 / 3.0 * M_PI * cutneigh * cutneigh * cutneigh;
+        int want = std::max(32, (int)std::ceil(2.0 * expect));
+        out.want_maxnbr = (want + 15) / 16 * 16;
+
+        // ---- Morton rank of interior cells (order of the sort key's bin part) ----
+        const int nc = d.ncell;
+        std::vector<std::pair<uint64_t, int>> order(nc);
+        for (int z = 0; z < d.nb[2]; ++z)
+            for (int y = 0; y < d.nb[1]; ++y)
+                for (int x = 0; x < d.nb[0]; ++x) {
+                    int lin = x + d.nb[0] * (y + d.nb[1] * z);
+                    order[lin] = {morton3(x + 1, y + 1, z + 1), lin};
+                }
+        std::sort(order.begin(), order.end());
+        out.rank.assign(nc + 1, 0);
+        out.xyz.assign(nc + 1, 0);
+        for (int r = 0; r < nc; ++r) {
+            int lin = order[r].second;
+            out.rank[lin] = r;
+            int x = lin % d.nb[0], y = (lin / d.nb[0]) % d.nb[1], z = lin / (d.nb[0] * d.nb[1]);
+            out.xyz[r] = x | (y << 10) | (z << 20);
+        }
+        out.mass.resize(ntypes);
+        out.minv.resize(ntypes);
+        for (int t = 0; t < ntypes; ++t) { out.mass[t] = (float)mass[t + 1]; out.minv[t] = (float)(1.0 / mass[t + 1]); }
+    }
+
+    // Particle arrays in the device layout, creation order, `count` slots
+    // (slots past n are inert padding).  origin = creation index.
+    void pack(int count, float4 *hp, float4 *hv, int *ht, int *hm, int *ho, int *himg) const
+    {
+        const int n = (int)h_tag.size();
+        for (int i = 0; i < count; ++i) {
+            if (i < n) {
+                int ty = h_type[i] - 1;
+                float tyf;
+                memcpy(&tyf, &ty, 4);
+                hp[i] = make_float4((float)h_x[3 * i], (float)h_x[3 * i + 1], (float)h_x[3 * i + 2], tyf);
+                hv[i] = make_float4((float)h_v[3 * i], (float)h_v[3 * i + 1], (float)h_v[3 * i + 2], 0.f);
+                ht[i] = h_tag[i];
+                hm[i] = h_mask[i];
+                ho[i] = i;
+                himg[i] = ((h_image[3 * i] + 512) & 1023) | (((h_image[3 * i + 1] + 512) & 1023) << 10) |
+                          (((h_image[3 * i + 2] + 512) & 1023) << 20);
+            } else {
+                hp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                hv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                ht[i] = 0; hm[i] = 0; ho[i] = -1;
+                himg[i] = 512 | (512 << 10) | (512 << 20);
+            }
+        }
+    }
+
+    void setup()
+    {
+        Desc ds;
+        describe(ds);
+        ensure_stream();
+        SysDev &d = ds.d;
+        maxnbr = std::max(maxnbr, ds.want_maxnbr);
+        d.maxnbr = maxnbr;
+        const int nc = d.ncell;
+        std::vector<int> csys(nc + 1, 0);
+
+        // ---- allocate ----
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        cap = d.npad;
+        ncell = nc;
+        nslots = nc + 1;
+        for (int b = 0; b < 2; ++b) {
+            pos[b].alloc(cap); vel[b].alloc(cap);
+            tag[b].alloc(cap); mask[b].alloc(cap); origin[b].alloc(cap); image[b].alloc(cap);
+        }
+        xhold.alloc(cap); frc.alloc(cap); vend.alloc(cap);
+        rho.alloc(cap); vir.alloc((size_t)6 * cap);
+        pcell.alloc(cap); pslot.alloc(cap); psub.alloc(cap); tmp.alloc(cap); key.alloc(2 * (size_t)cap);
+        ncount.alloc(cap); ccount.alloc((size_t)4 * cap);
+        nlist.alloc((size_t)cap * maxnbr);
+        clist.alloc((size_t)cap * maxnbr);
+        wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE); status.alloc(1);
+        cl.alloc(cap); wl.alloc(cap); work_total.alloc(1);
+        cell_count.alloc(nslots); cell_start.alloc(nslots); cell_xyz.alloc(nslots);
+        cell_rank.alloc(nslots); cell_sys.alloc(nslots);
+        coef.alloc(ds.coef.size()); minv.alloc(ntypes); mmass.alloc(ntypes); dsys.alloc(1);
+
+        // ---- upload ----
+        std::vector<float4> hp(cap), hv(cap);
+        std::vector<int> ht(cap), hm(cap), ho(cap), himg(cap);
+        pack(cap, hp.data(), hv.data(), ht.data(), hm.data(), ho.data(), himg.data());
+        std::vector<int> ws(cap / WAVE, 0);
+        auto up = [&](void *dst, const void *src, size_t bytes) {
+            hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "upload");
+        };
+        up(pos[0].p, hp.data(), cap * sizeof(float4));
+        up(vel[0].p, hv.data(), cap * sizeof(float4));
+        up(tag[0].p, ht.data(), cap * sizeof(int));
+        up(mask[0].p, hm.data(), cap * sizeof(int));
+        up(origin[0].p, ho.data(), cap * sizeof(int));
+        up(image[0].p, himg.data(), cap * sizeof(int));
+        up(wave_sys.p, ws.data(), ws.size() * sizeof(int));
+        up(cell_rank.p, ds.rank.data(), nslots * sizeof(int));
+        up(cell_xyz.p, ds.xyz.data(), nslots * sizeof(int));
+        up(cell_sys.p, csys.data(), nslots * sizeof(int));
+        up(coef.p, ds.coef.data(), ds.coef.size() * sizeof(float));
+        up(minv.p, ds.minv.data(), ntypes * sizeof(float));
+        up(mmass.p, ds.mass.data(), ntypes * sizeof(float));
+        hip_check(hipMemsetAsync(cell_count.p, 0, nslots * sizeof(int), stream), "memset");
+        hip_check(hipMemsetAsync(work_total.p, 0, sizeof(int), stream), "memset");
+        hip_check(hipMemsetAsync(ncount.p, 0, cap * sizeof(int), stream), "memset");
+        hip_check(hipMemsetAsync(rho.p, 0, cap * sizeof(float), stream), "memset");
+        hip_check(hipMemsetAsync(frc.p, 0, cap * sizeof(float4), stream), "memset");
+        hip_check(hipMemsetAsync(vend.p, 0, cap * sizeof(float4), stream), "memset");
+        hip_check(hipMemsetAsync(vir.p, 0, 6 * cap * sizeof(float), stream), "memset");
+        hsys = d;
+        up(dsys.p, &hsys, sizeof(SysDev));
+
+        rebuild();
+        PoolDev P = view();
+        hip_check(launch_signatures(P, stream), "signatures");
+        force_pass(MODE_SETUP);
+        hip_check(hipStreamSynchronize(stream), "setup");
+        collect_events();
+        setup_done = true;
+        dirty = false;
+    }
+
+    // rebuild phase for a stale system; grows the neighbor rows on overflow
+    void rebuild()
+    {
+        PoolDev P = view();
+        timed(PH_REBUILD, [&]() { hip_check(launch_rebuild(P, stream), "rebuild"); });
+        for (int attempt = 0;; ++attempt) {
+            int ovf = read_field(offsetof(SysDev, overflow));
+            if (!ovf) break;
+            if (attempt > 4) fail(MDPD_E_OVERFLOW, "<MESO> Pair table overflow; local density too high");
+            // grow rows and rebuild the rows only (sorted data is in place)
+            maxnbr = (int)std::ceil(maxnbr * 1.5 / 16.0) * 16;
+            nlist.alloc((size_t)cap * maxnbr);
+            clist.alloc((size_t)cap * maxnbr);
+            int zero = 0;
+            hip_check(hipMemcpyAsync((char *)dsys.p + offsetof(SysDev, overflow), &zero, sizeof(int),
+                                     hipMemcpyHostToDevice, stream), "clear overflow");
+            hip_check(hipMemcpyAsync((char *)dsys.p + offsetof(SysDev, maxnbr), &maxnbr, sizeof(int),
+                                     hipMemcpyHostToDevice, stream), "maxnbr");
+            P = view();
+            hip_check(launch_nlist_only(P, stream), "nlist");
+        }
+        hip_check(launch_commit_rebuild(P, stream), "commit");
+        ++nbuild;
+    }
+
+    hipEvent_t take_event()
+    {
+        if (!ev_free.empty()) { hipEvent_t e = ev_free.back(); ev_free.pop_back(); return e; }
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+    template <typename F>
+    void timed(int phase, F launch)
+    {
+        if (!prof) { launch(); return; }
+        hipEvent_t a = take_event(), b = take_event();
+        hip_check(hipEventRecord(a, stream), "hipEventRecord");
+        launch();
+        hip_check(hipEventRecord(b, stream), "hipEventRecord");
+        ev_open.push_back({phase, {a, b}});
+    }
+    // fold finished event pairs into the per-phase totals (stream synchronized)
+    void collect_events()
+    {
+        for (auto &e : ev_open) {
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, e.second.first, e.second.second), "hipEventElapsedTime");
+            prof_ms[e.first] += ms;
+            prof_n[e.first] += 1;
+            ev_free.push_back(e.second.first);
+            ev_free.push_back(e.second.second);
+        }
+        ev_open.clear();
+    }
+
+    int read_field(size_t offset)
+    {
+        hip_check(hipMemcpyAsync(pinned, (char *)dsys.p + offset, sizeof(int), hipMemcpyDeviceToHost, stream),
+                  "readback");
+        hip_check(hipStreamSynchronize(stream), "readback sync");
+        return pinned[0];
+    }
+
+    void force_pass(int mode)
+    {
+        PoolDev P = view();
+        const bool with_rho = pair_style == STYLE_MDPD && hsys.rho_bit;
+        timed(PH_DENSITY, [&]() { hip_check(launch_first(P, with_rho, split, stream), "first pass"); });
+        timed(PH_FORCE, [&]() { hip_check(launch_force(P, pair_style, mode, ntypes == 1, split, stream), "force"); });
+        if (mode != MODE_SETUP) hip_check(launch_commit_step(P, mode != MODE_FINAL, split, stream), "commit");
+    }
+
+    // LAMMPS Neighbor::decide with the flag computed on the device
+    bool want_rebuild(long long ago, int stale)
+    {
+        if (!(ago >= delay && ago % every == 0)) return false;
+        if (!check) return true;
+        return stale != 0;
+    }
+
+    void run(long long nsteps)
+    {
+        if (nsteps < 0) fail(MDPD_E_ARG, "invalid run length");
+        if (!run_style_ok) fail(MDPD_E_STATE, "run_style mvv/meso must be set for the USER-MESO styles");
+        if (!setup_done || dirty) setup();
+        if (nsteps == 0) return;
+        PoolDev P = view();
+        timed(PH_INITIAL, [&]() { hip_check(launch_initial(P, stream), "initial"); });
+        hip_check(launch_commit_step(P, 1, split, stream), "commit");
+        long long ago = 0;
+        for (long long k = 1; k <= nsteps; ++k) {
+            ++ago;
+            int stale = read_field(offsetof(SysDev, stale));
+            if (want_rebuild(ago, stale)) {
+                if (!stale) {   // forced by every/delay without check
+                    int one = 1;
+                    hip_check(hipMemcpyAsync((char *)dsys.p + offsetof(SysDev, stale), &one, sizeof(int),
+                                             hipMemcpyHostToDevice, stream), "stale");
+                }
+                rebuild();
+                ago = 0;
+            }
+            force_pass(k < nsteps ? MODE_FUSED : MODE_FINAL);
+        }
+        hip_check(hipStreamSynchronize(stream), "run");
+        collect_events();
+        ntimestep += nsteps;
+    }
+
+    // ------------------------------------------------------------------
+    // gather
+    // ------------------------------------------------------------------
+    template <typename T>
+    std::vector<T> download(const T *src, size_t count)
+    {
+        std::vector<T> h(count);
+        if (count)
+            hip_check(hipMemcpyAsync(h.data(), src, count * sizeof(T), hipMemcpyDeviceToHost, stream), "download");
+        return h;
+    }
+
+    void sync_sys()
+    {
+        hip_check(hipMemcpyAsync(&hsys, dsys.p, sizeof(SysDev), hipMemcpyDeviceToHost, stream), "sys");
+        hip_check(hipStreamSynchronize(stream), "sync");
+    }
+
+    void gather(const std::string &name, int type, int count, void *data)
+    {
+        const int n = (int)h_tag.size();
+        if (!setup_done) {
+            // creation-order host copies before the first setup
+            if (type == 1 && count == 3 && (name == "x" || name == "v")) {
+                memcpy(data, (name == "x" ? h_x : h_v).data(), sizeof(double) * 3 * n);
+                return;
+            }
+            if (type == 0 && count == 1 && (name == "tag" || name == "type" || name == "mask")) {
+                const std::vector<int> &s = name == "tag" ? h_tag : (name == "type" ? h_type : h_mask);
+                memcpy(data, s.data(), sizeof(int) * n);
+                return;
+            }
+            fail(MDPD_E_STATE, "%s is only available after setup", name.c_str());
+        }
+        sync_sys();
+        const int sel = hsys.sel, sela = hsys.sela;
+        std::vector<int> org = download(origin[sela].p, cap);
+        hip_check(hipStreamSynchronize(stream), "sync");
+        auto put_d = [&](int k, int c, double v) { ((double *)data)[(size_t)org[k] * count + c] = v; };
+        auto put_i = [&](int k, int c, int v) { ((int *)data)[(size_t)org[k] * count + c] = v; };
+        auto want = [&](int t, int c) {
+            if (type != t || count != c) fail(MDPD_E_ARG, "gather of %s needs type %d count %d", name.c_str(), t, c);
+        };
+        if (name == "x" || name == "v" || name == "f") {
+            want(1, 3);
+            std::vector<float4> h = download(name == "x" ? pos[sel].p : (name == "v" ? vel[sel].p : frc.p), cap);
+            hip_check(hipStreamSynchronize(stream), "sync");
+            for (int k = 0; k < n; ++k) { put_d(k, 0, h[k].x); put_d(k, 1, h[k].y); put_d(k, 2, h[k].z); }
+        } else if (name == "e_pair") {
+            want(1, 1);
+            std::vector<float4> h = download(frc.p, cap);
+            hip_check(hipStreamSynchronize(stream), "sync");
+            for (int k = 0; k < n; ++k) put_d(k, 0, h[k].w);
+        } else if (name == "rho") {
+            want(1, 1);
+            std::vector<float> h = download(rho.p, cap);
+            hip_check(hipStreamSynchronize(stream), "sync");
+            for (int k = 0; k < n; ++k) put_d(k, 0, h[k]);
+        } else if (name == "virial") {
+            want(1, 6);
+            std::vector<float> h = download(vir.p, (size_t)6 * cap);
+            hip_check(hipStreamSynchronize(stream), "sync");
+            for (int k = 0; k < n; ++k)
+                for (int c = 0; c < 6; ++c) put_d(k, c, h[(size_t)c * cap + k]);
+        } else if (name == "tag" || name == "mask") {
+            want(0, 1);
+            std::vector<int> h = download(name == "tag" ? tag[sela].p : mask[sela].p, cap);
+            hip_check(hipStreamSynchronize(stream), "sync");
+            for (int k = 0; k < n; ++k) put_i(k, 0, h[k]);
+        } else if (name == "type") {
+            want(0, 1);
+            std::vector<float4> h = download(pos[sel].p, cap);
+            hip_check(hipStreamSynchronize(stream), "sync");
+            for (int k = 0; k < n; ++k) { int t; memcpy(&t, &h[k].w, 4); put_i(k, 0, t + 1); }
+        } else if (name == "image") {
+            want(0, 3);
+            std::vector<int> h = download(image[sela].p, cap);
+            hip_check(hipStreamSynchronize(stream), "sync");
+            for (int k = 0; k < n; ++k) {
+                put_i(k, 0, (h[k] & 1023) - 512);
+                put_i(k, 1, ((h[k] >> 10) & 1023) - 512);
+                put_i(k, 2, ((h[k] >> 20) & 1023) - 512);
+            }
+        } else {
+            fail(MDPD_E_ARG, "unknown per-atom quantity '%s'", name.c_str());
+        }
+    }
+
+    // pull the device state back into the creation-order host arrays
+    void pull_state()
+    {
+        if (!setup_done) return;
+        std::vector<double> x(3 * h_tag.size()), v(3 * h_tag.size());
+        std::vector<int> im(3 * h_tag.size());
+        gather("x", 1, 3, x.data());
+        gather("v", 1, 3, v.data());
+        gather("image", 0, 3, im.data());
+        h_x = x;
+        h_v = v;
+        h_image = im;
+    }
+};

@@ -32,7 +32,7 @@ import os
 
 import numpy as np
 
-__all__ = ["System", "MdpdError", "tea_gaussian", "tea_premix", "library_path", "load"]
+__all__ = ["System", "Pool", "MdpdError", "tea_gaussian", "tea_premix", "clock", "library_path", "load"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBNAME = "libmdpd355.so"
@@ -91,6 +91,21 @@ def load():
         "mdpd_debug_signatures": (I, [P, P]),
         "mdpd_tea_gaussian": (I, [I, I, P, P, P]),
         "mdpd_tea_premix": (I, [I, I, P, P, I, P]),
+        "mdpd_pool_open": (I, [I, D, I, ctypes.POINTER(P)]),
+        "mdpd_pool_close": (None, [P]),
+        "mdpd_pool_last_error": (ctypes.c_char_p, [P]),
+        "mdpd_pool_submit": (I, [P, P, LL, ctypes.POINTER(I)]),
+        "mdpd_pool_advance": (I, [P, LL, I, ctypes.POINTER(I)]),
+        "mdpd_pool_run": (I, [P]),
+        "mdpd_pool_job_state": (I, [P, I, ctypes.POINTER(I)]),
+        "mdpd_pool_done": (I, [P, P, I, ctypes.POINTER(I)]),
+        "mdpd_pool_gather": (I, [P, I, ctypes.c_char_p, I, I, P]),
+        "mdpd_pool_thermo": (I, [P, I, ctypes.POINTER(D)]),
+        "mdpd_pool_job_info": (I, [P, I, ctypes.POINTER(LL), ctypes.POINTER(LL), ctypes.POINTER(D)]),
+        "mdpd_pool_release": (I, [P, I]),
+        "mdpd_pool_tune": (I, [P, ctypes.c_char_p, I]),
+        "mdpd_pool_stat": (I, [P, ctypes.c_char_p, ctypes.POINTER(D)]),
+        "mdpd_clock": (D, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -242,6 +257,105 @@ class System:
         out = np.empty(self.natoms, dtype=np.uint32)
         self._check(self._lib.mdpd_debug_signatures(self._h, _ptr(out)))
         return out
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
+
+
+class Pool:
+    """Pooled engine: many independent systems advanced together, one launch
+    sequence per pass (batched runs and served jobs).  Jobs are submitted from
+    configured System handles, whose state and atoms are copied."""
+
+    def __init__(self, hbm_budget_bytes, max_systems, device=0):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        rc = self._lib.mdpd_pool_open(int(device), float(hbm_budget_bytes), int(max_systems), ctypes.byref(h))
+        if rc != 0:
+            raise MdpdError(rc, "mdpd_pool_open failed")
+        self._h = h
+        self._natoms = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MdpdError(rc, self._lib.mdpd_pool_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mdpd_pool_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, system, nsteps):
+        j = ctypes.c_int()
+        self._check(self._lib.mdpd_pool_submit(self._h, system._h, int(nsteps), ctypes.byref(j)))
+        self._natoms[j.value] = system.natoms
+        return j.value
+
+    def advance(self, max_passes=0, stop_on_done=False):
+        d = ctypes.c_int()
+        self._check(self._lib.mdpd_pool_advance(self._h, int(max_passes), int(bool(stop_on_done)), ctypes.byref(d)))
+        return d.value
+
+    def run(self):
+        self._check(self._lib.mdpd_pool_run(self._h))
+
+    def state(self, job):
+        st = ctypes.c_int()
+        self._check(self._lib.mdpd_pool_job_state(self._h, int(job), ctypes.byref(st)))
+        return st.value
+
+    def done(self, max_jobs=4096):
+        ids = np.empty(max_jobs, dtype=np.int32)
+        c = ctypes.c_int()
+        self._check(self._lib.mdpd_pool_done(self._h, _ptr(ids), int(max_jobs), ctypes.byref(c)))
+        return ids[: c.value].tolist()
+
+    def gather(self, job, name):
+        t, c, dt = _FIELDS[name]
+        out = np.empty((self._natoms[job], c), dtype=dt)
+        self._check(self._lib.mdpd_pool_gather(self._h, int(job), name.encode(), t, c, _ptr(out)))
+        return out[:, 0].copy() if c == 1 else out
+
+    def thermo(self, job):
+        out = (ctypes.c_double * 4)()
+        self._check(self._lib.mdpd_pool_thermo(self._h, int(job), out))
+        return {"temp": out[0], "press": out[1], "pe": out[2], "ke": out[3]}
+
+    def info(self, job):
+        n = ctypes.c_longlong()
+        ts = ctypes.c_longlong()
+        t = (ctypes.c_double * 3)()
+        self._check(self._lib.mdpd_pool_job_info(self._h, int(job), ctypes.byref(n), ctypes.byref(ts), t))
+        return {"natoms": n.value, "timestep": ts.value, "t_submit": t[0], "t_admit": t[1], "t_done": t[2]}
+
+    def release(self, job):
+        self._check(self._lib.mdpd_pool_release(self._h, int(job)))
+        self._natoms.pop(job, None)
+
+    def tune(self, knob, value):
+        self._check(self._lib.mdpd_pool_tune(self._h, knob.encode(), int(value)))
+
+    def stat(self, name):
+        v = ctypes.c_double()
+        self._check(self._lib.mdpd_pool_stat(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+
+def clock():
+    """The engine's monotonic clock (seconds), the time base of job times."""
+    return load().mdpd_clock()
 
 
 def tea_gaussian(a, b, device=0):
