@@ -183,6 +183,9 @@ int mdpd_pool_submit(mdpd_pool *pool, mdpd_system *sys, long long nsteps, int *j
 int mdpd_pool_advance(mdpd_pool *pool, long long max_passes, int stop_on_done, int *ndone);
 /* Run until every submitted job is done. */
 int mdpd_pool_run(mdpd_pool *pool);
+/* Admit the queued jobs that fit now and wait until their state is resident
+ * on the device (no pass is run); *admitted = jobs admitted. */
+int mdpd_pool_admit(mdpd_pool *pool, int *admitted);
 int mdpd_pool_job_state(mdpd_pool *pool, int job, int *state);
 /* Ids of jobs finished since the last call (up to max). */
 int mdpd_pool_done(mdpd_pool *pool, int *jobs, int max, int *count);
@@ -199,11 +202,14 @@ int mdpd_pool_job_info(mdpd_pool *pool, int job, long long *natoms, long long *t
 int mdpd_pool_release(mdpd_pool *pool, int job);
 /* "split" 0 (autotune on the first passes, the default), 1, 2 or 4;
  * "graph" 1/0: replay each pass as a captured hipGraph (default 1);
- * "poll": passes between status reads (default 8). */
+ * "poll": passes between status reads (default 8);
+ * "profile" N: every N-th batch of passes is launched without the graph with
+ * HIP events between its phases (0: off, the default; also resets the totals). */
 int mdpd_pool_tune(mdpd_pool *pool, const char *knob, int value);
 /* Statistics: passes, particle_steps (of finished jobs), split, capacity
  * (particle slots), maxnbr, bytes (device bytes held), regrows, running,
- * queued, used_slots, tune_ms_1 / tune_ms_2 / tune_ms_4 (ms per pass). */
+ * queued, used_slots, tune_ms_1 / tune_ms_2 / tune_ms_4 (ms per pass),
+ * prof_passes and prof_{rebuild,first,force,commit}_ms (sampled totals). */
 int mdpd_pool_stat(mdpd_pool *pool, const char *name, double *value);
 /* Monotonic clock of the job times, in seconds. */
 double mdpd_clock(void);

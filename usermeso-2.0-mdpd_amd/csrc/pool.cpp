@@ -190,6 +190,17 @@ struct mdpd_pool {
     long long passes = 0, regrows = 0;
     double particle_steps = 0;
 
+    // ---- sampled kernel timing: every prof_every-th batch of passes is
+    // launched without the graph, with events between the phases ----
+    int prof_every = 0;
+    long long batches = 0;
+    enum { PP_REBUILD = 0, PP_FIRST, PP_FORCE, PP_COMMIT, PP_COUNT };
+    struct PassEvents { hipEvent_t e[PP_COUNT + 1]; };
+    std::vector<PassEvents> prof_open;
+    std::vector<hipEvent_t> prof_free;
+    double prof_ms[PP_COUNT] = {0, 0, 0, 0};
+    long long prof_passes = 0;
+
     ~mdpd_pool()
     {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -203,6 +214,9 @@ struct mdpd_pool {
             if (j.second->stage) (void)hipHostFree(j.second->stage);
         }
         for (auto e : ev_free) (void)hipEventDestroy(e);
+        for (auto &pe : prof_open)
+            for (auto e : pe.e) (void)hipEventDestroy(e);
+        for (auto e : prof_free) (void)hipEventDestroy(e);
         for (int k = 0; k < 2; ++k) {
             if (ev_poll[k]) (void)hipEventDestroy(ev_poll[k]);
             if (pin_status[k]) (void)hipHostFree(pin_status[k]);
@@ -525,14 +539,55 @@ struct mdpd_pool {
     // ------------------------------------------------------------------
     // passes
     // ------------------------------------------------------------------
-    void launch_pass(int sp)
+    hipEvent_t timing_event()
+    {
+        if (!prof_free.empty()) {
+            hipEvent_t e = prof_free.back();
+            prof_free.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+
+    // one pass; with `ev`, events are recorded between its phases
+    void launch_pass(int sp, PassEvents *ev = nullptr)
     {
         PoolDev P = view();
+        auto mark = [&](int k) {
+            if (ev) hip_check(hipEventRecord(ev->e[k], stream), "hipEventRecord");
+        };
+        mark(PP_REBUILD);
         hip_check(launch_rebuild(P, stream), "rebuild");
         hip_check(launch_commit_rebuild(P, stream), "commit rebuild");
+        mark(PP_FIRST);
         hip_check(launch_first(P, with_rho, sp, stream), "first pass");
+        mark(PP_FORCE);
         for (auto &c : combos) hip_check(launch_force(P, c.first, MODE_PHASED, c.second != 0, sp, stream), "force");
+        mark(PP_COMMIT);
         hip_check(launch_commit_step(P, 2, sp, stream), "commit step");
+        mark(PP_COUNT);
+    }
+
+    // fold finished sampled passes into the per-phase totals
+    void collect_profile(bool block)
+    {
+        for (size_t k = 0; k < prof_open.size();) {
+            PassEvents &pe = prof_open[k];
+            const hipError_t q = block ? hipEventSynchronize(pe.e[PP_COUNT]) : hipEventQuery(pe.e[PP_COUNT]);
+            if (q == hipErrorNotReady) { ++k; continue; }
+            hip_check(q, "profile event");
+            for (int ph = 0; ph < PP_COUNT; ++ph) {
+                float ms = 0;
+                hip_check(hipEventElapsedTime(&ms, pe.e[ph], pe.e[ph + 1]), "hipEventElapsedTime");
+                prof_ms[ph] += ms;
+            }
+            ++prof_passes;
+            for (auto e : pe.e) prof_free.push_back(e);
+            prof_open[k] = prof_open.back();
+            prof_open.pop_back();
+        }
     }
 
     void capture()
@@ -560,7 +615,15 @@ struct mdpd_pool {
     void run_passes(int m)
     {
         if (m <= 0) return;
-        if (use_graph) {
+        const bool sample = prof_every > 0 && batches++ % prof_every == 0;
+        if (sample) {
+            for (int k = 0; k < m; ++k) {
+                PassEvents pe;
+                for (auto &e : pe.e) e = timing_event();
+                launch_pass(split, &pe);
+                prof_open.push_back(pe);
+            }
+        } else if (use_graph) {
             if (!graph_valid) capture();
             for (int k = 0; k < m; ++k) hip_check(hipGraphLaunch(gexec, stream), "hipGraphLaunch");
         } else {
@@ -636,6 +699,7 @@ struct mdpd_pool {
 
     void handle(const int *st)
     {
+        collect_profile(false);
         for (int s = 0; s < max_sys; ++s) {
             const int jid = slot_job[s];
             if (!jid) continue;
@@ -802,8 +866,21 @@ struct mdpd_pool {
         if (nrunning == 0 && nharvest == 0 && queue.empty()) {
             drain_polls();
             recycle_uploads(true);
+            collect_profile(true);
         }
         return ndone;
+    }
+
+    // admit what fits now and wait until those jobs are resident
+    int admit_now()
+    {
+        ensure_stream();
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        const int before = nrunning;
+        admit();
+        hip_check(hipStreamSynchronize(stream), "admit");
+        recycle_uploads(true);
+        return nrunning - before;
     }
 
     Job &job(int id)
@@ -871,6 +948,14 @@ int mdpd_pool_advance(mdpd_pool *pool, long long max_passes, int stop_on_done, i
 }
 
 int mdpd_pool_run(mdpd_pool *pool) { POOL_GUARD(pool, { pool->advance(0, false); }); }
+
+int mdpd_pool_admit(mdpd_pool *pool, int *admitted)
+{
+    POOL_GUARD(pool, {
+        const int a = pool->admit_now();
+        if (admitted) *admitted = a;
+    });
+}
 
 int mdpd_pool_job_state(mdpd_pool *pool, int job, int *state)
 {
@@ -966,6 +1051,11 @@ int mdpd_pool_tune(mdpd_pool *pool, const char *knob, int value)
             pool->graph_valid = false;
         } else if (k == "graph") {
             pool->use_graph = value != 0;
+        } else if (k == "profile") {
+            if (value < 0) fail(MDPD_E_ARG, "profile must be >= 0");
+            pool->prof_every = value;
+            for (double &m : pool->prof_ms) m = 0;
+            pool->prof_passes = 0;
         } else if (k == "poll") {
             if (value < 1 || value > 4096) fail(MDPD_E_ARG, "poll must be 1..4096 passes");
             pool->passes_per_poll = value;
@@ -990,6 +1080,11 @@ int mdpd_pool_stat(mdpd_pool *pool, const char *name, double *value)
         else if (k == "running") *value = pool->nrunning + pool->nharvest;
         else if (k == "queued") *value = (double)pool->queue.size();
         else if (k == "used_slots") *value = (double)pool->parts.used * 256.0;
+        else if (k == "prof_passes") *value = (double)pool->prof_passes;
+        else if (k == "prof_rebuild_ms") *value = pool->prof_ms[mdpd_pool::PP_REBUILD];
+        else if (k == "prof_first_ms") *value = pool->prof_ms[mdpd_pool::PP_FIRST];
+        else if (k == "prof_force_ms") *value = pool->prof_ms[mdpd_pool::PP_FORCE];
+        else if (k == "prof_commit_ms") *value = pool->prof_ms[mdpd_pool::PP_COMMIT];
         else if (k == "tune_ms_1") *value = pool->tune_ms[1];
         else if (k == "tune_ms_2") *value = pool->tune_ms[2];
         else if (k == "tune_ms_4") *value = pool->tune_ms[4];

@@ -97,6 +97,7 @@ def load():
         "mdpd_pool_submit": (I, [P, P, LL, ctypes.POINTER(I)]),
         "mdpd_pool_advance": (I, [P, LL, I, ctypes.POINTER(I)]),
         "mdpd_pool_run": (I, [P]),
+        "mdpd_pool_admit": (I, [P, ctypes.POINTER(I)]),
         "mdpd_pool_job_state": (I, [P, I, ctypes.POINTER(I)]),
         "mdpd_pool_done": (I, [P, P, I, ctypes.POINTER(I)]),
         "mdpd_pool_gather": (I, [P, I, ctypes.c_char_p, I, I, P]),
@@ -310,6 +311,11 @@ class Pool:
 
     def run(self):
         self._check(self._lib.mdpd_pool_run(self._h))
+
+    def admit(self):
+        a = ctypes.c_int()
+        self._check(self._lib.mdpd_pool_admit(self._h, ctypes.byref(a)))
+        return a.value
 
     def state(self, job):
         st = ctypes.c_int()
