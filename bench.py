@@ -19,9 +19,11 @@ log-uniform in [1000, 200000], `--steps` steps, admitted by the pool's HBM
 budget while others run; it reports throughput and p50/p99 latency from
 arrival to result.
 
-`--workload slab` runs the plain mDPD liquid slab instead (box 40 x 80 x 20,
+Other workloads on the plain path (one handle): `slab` (box 40 x 80 x 20,
 slab z in [5, 15) at density 6.05, N = 193,600, Lucy weight; `slab-literal`:
-40 x 40 x 20, N = 96,800) on one handle.
+40 x 40 x 20, N = 96,800), `pore` (quasi-2D pore flow, 200 x 100 x 4, frozen
+disc walls, body force; ~480,000 particles) and `dpd` (classic DPD fluid,
+L = 22, N = 32,000, beside the CPU LAMMPS pair_style dpd transcription).
 
 Prints one JSON line.
 """
@@ -105,6 +107,40 @@ def make_slab_job(n, seed):
     x[:, 1] = rng.uniform(0, side, n)
     x[:, 2] = rng.uniform(5.0, 15.0, n)
     return box, _f32(x), _f32(_mb(rng, n))
+
+
+def make_pore(seed=20240603, lx=200.0, ly=100.0, lz=4.0, cover=0.40, rho=6.0):
+    """Quasi-2D pore medium: seeded non-overlapping discs (radius U(4, 10),
+    periodic in x and y) placed until they cover 40% of the x-y area, extruded
+    in z; particles uniform at density 6, type 1 (frozen wall) inside a disc,
+    type 2 (fluid) elsewhere."""
+    rng = np.random.default_rng(seed)
+    discs, area = [], 0.0
+    while area < cover * lx * ly:
+        r = rng.uniform(4.0, 10.0)
+        c = rng.uniform((0, 0), (lx, ly))
+        ok = True
+        for (q, rq) in discs:
+            d = c - q
+            d -= np.array([lx, ly]) * np.round(d / np.array([lx, ly]))
+            if np.hypot(*d) < r + rq:
+                ok = False
+                break
+        if ok:
+            discs.append((c, r))
+            area += np.pi * r * r
+    n = int(round(rho * lx * ly * lz))
+    x = rng.uniform(0, 1, (n, 3)) * np.array([lx, ly, lz])
+    wall = np.zeros(n, bool)
+    for (q, r) in discs:
+        d = x[:, :2] - q
+        d -= np.array([lx, ly]) * np.round(d / np.array([lx, ly]))
+        wall |= (d * d).sum(axis=1) < r * r
+    types = np.where(wall, 1, 2).astype(np.int32)
+    v = _mb(rng, n)
+    v[wall] = 0.0
+    v[~wall] -= v[~wall].mean(axis=0)
+    return (lx, ly, lz), _f32(x), _f32(v), types, len(discs)
 
 
 def batch_sizes(jobs, seed):
@@ -336,6 +372,129 @@ def run_batched(a):
     print(json.dumps(base_line(a, value, dt / max(st["passes"], 1) * 1e3, config, roof, cpu)))
 
 
+def run_pore(a):
+    import mdpd355
+    box, x, v, types, ndisc = make_pore()
+    n = len(x)
+    nwall = int((types == 1).sum())
+
+    def handle():
+        s = mdpd355.System(0)
+        s.command("atom_style mdpd/atomic/meso 6.0")
+        s.create_box(2, (0, 0, 0), box)
+        s.create_atoms(types, x, v)
+        s.commands(f"""
+            neighbor {SKIN} bin
+            neigh_modify every 1 delay 0 check yes
+            mass * 1.0
+            group fluid type 2
+            pair_style mdpd/meso 1.0 {RC} 4242
+            pair_coeff 1 1 -40 25 4.5 {RC} {RD}
+            pair_coeff 1 2 -40 0 4.5 {RC} {RD}
+            pair_coeff 2 2 -40 25 4.5 {RC} {RD}
+            fix rho all aboundary/rho/meso cut_rho {RD}
+            fix push fluid addforce/meso 0.02 0 0
+            fix int fluid nve/meso
+            timestep 0.01
+            run_style mvv/meso
+        """)
+        if a.split:
+            s.tune("split", a.split)
+        return s
+
+    if a.warmup > 0:
+        s = handle()
+        s.run(a.warmup)
+        s.close()
+    s = handle()
+    s.setup()
+    s.profile(True)
+    t0 = time.perf_counter()
+    s.run(a.steps)
+    dt = time.perf_counter() - t0
+    prof = {ph: s.profile_get(ph) for ph in ("force", "density", "rebuild", "initial")}
+    vx = s.gather("v")[types == 2, 0].mean()
+    nb = s.nbuild
+    s.close()
+    value = n * a.steps / dt
+    f_ms, f_n = prof["force"]
+    config = {
+        "workload": f"plain mDPD quasi-2D pore flow: box 200x100x4, {ndisc} seeded non-overlapping discs "
+                    f"(radius U(4,10)) covering 40% of the area, frozen wall particles (type 1, outside the NVE "
+                    f"group, in the density) and fluid (type 2) at density 6, N={n} ({nwall} wall), A=-40, "
+                    f"B=25 (solid-fluid B=0), gamma=4.5, rc=1, rd=0.75, g_x=0.02 on the fluid, dt=0.01, "
+                    f"skin 0.3, one system per launch",
+        "mode": "plain", "particles": n, "wall_particles": nwall, "builds": nb,
+        "phase_ms_per_step": {k: ms / max(a.steps, 1) for k, (ms, _) in prof.items()},
+        "fluid_mean_vx_end": float(vx),
+        "step_algorithmic_GBps": value * B_STEP / 1e9,
+    }
+    roof = roofline(f_ms / max(f_n, 1) * 1e-3, n, "k_force<mDPD, fused velocity-Verlet, 2 types>")
+    print(json.dumps(base_line(a, value, dt / a.steps * 1e3, config, roof, None)))
+
+
+def run_dpd(a):
+    """Config 1: the classic DPD fluid (L = 22, N = 32,000, a = 25, gamma = 4.5,
+    dt = 0.04) on the GPU (dpd/meso, sigma = 3) beside the CPU LAMMPS
+    pair_style dpd + fix nve transcription (serial: one Marsaglia stream)."""
+    import mdpd355
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import orc
+    L, n = 22.0, 32000
+    rng = np.random.default_rng(a.seed)
+    x = _f32(rng.uniform(0, L, (n, 3)))
+    v = _f32(_mb(rng, n))
+    s = mdpd355.System(0)
+    s.command("atom_style dpd/atomic/meso")
+    s.create_box(1, (0, 0, 0), (L, L, L))
+    s.create_atoms(np.ones(n, np.int32), x, v)
+    s.commands("""
+        neighbor 0.3 bin
+        neigh_modify every 1 delay 0 check yes
+        mass 1 1.0
+        pair_style dpd/meso 1.0 4242
+        pair_coeff * * 25.0 4.5 3.0 1.0 1.0
+        fix int all nve/meso
+        timestep 0.04
+        run_style mvv/meso
+    """)
+    s.run(a.warmup)
+    s.profile(True)
+    t0 = time.perf_counter()
+    s.run(a.steps)
+    dt = time.perf_counter() - t0
+    prof = {ph: s.profile_get(ph) for ph in ("force", "density", "rebuild", "initial")}
+    th = s.thermo()
+    s.close()
+    value = n * a.steps / dt
+    cpu = None
+    if not a.no_cpu:
+        steps = 2
+        t0 = time.perf_counter()
+        orc.lammps_dpd_run(x, v, (0, 0, 0), (L, L, L), nsteps=steps, dt=0.04, seed=4321)
+        per = (time.perf_counter() - t0) / steps
+        steps = int(max(2, min(1000, 12.0 / per)))
+        t0 = time.perf_counter()
+        out = orc.lammps_dpd_run(x, v, (0, 0, 0), (L, L, L), nsteps=steps, dt=0.04, seed=4321)
+        t = time.perf_counter() - t0
+        cpu = {"value": n * steps / t, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"LAMMPS pair_style dpd + fix nve transcription (FP64, half list, serial RanMars) on "
+                         f"the same N={n} fluid, {steps} steps incl. the initial build, {t:.1f} s wall"}
+    b_dpd = 56 + 4 * 3.0 * 4.0 / 3.0 * np.pi * 1.3**3
+    f_ms, f_n = prof["force"]
+    config = {
+        "workload": f"plain DPD fluid (config 1): periodic cube L=22, N={n} uniform, a=25 gamma=4.5 sigma=3 "
+                    f"rc=1 dt=0.04, skin 0.3, dpd/meso on one handle",
+        "mode": "plain", "particles": n, "final_temp": th["temp"], "final_press": th["press"],
+        "phase_ms_per_step": {k: ms / max(a.steps, 1) for k, (ms, _) in prof.items()},
+    }
+    roof = {"bound": "hbm", "kernel": "k_force<DPD, fused velocity-Verlet>",
+            "achieved": b_dpd * n / (f_ms / max(f_n, 1) * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "traffic": None, "bytes_per_particle": b_dpd}
+    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS
+    print(json.dumps(base_line(a, value, dt / a.steps * 1e3, config, roof, cpu)))
+
+
 def run_slab(a):
     w = SLABS[a.workload]
     box = w["box"]
@@ -367,7 +526,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000, help="steps per job")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="batched", choices=["batched"] + sorted(SLABS))
+    ap.add_argument("--workload", default="batched", choices=["batched", "pore", "dpd"] + sorted(SLABS))
     ap.add_argument("--jobs", type=int, default=256)
     ap.add_argument("--seed", type=int, default=20241202)
     ap.add_argument("--budget-gb", type=float, default=16.0, help="HBM budget of the batched pool")
@@ -377,13 +536,28 @@ def main():
     ap.add_argument("--served-budget-gb", type=float, default=4.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--split", type=int, default=None, help="waves per tile (default: autotuned / 2)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device with a plain copy kernel (a known-size "
+                         "reference for HBM byte counters under rocprofv3 --pmc)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+    if a.calibrate_copy_mib > 0:
+        import torch
+        src = torch.ones(a.calibrate_copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copied {a.calibrate_copy_mib} MiB", file=sys.stderr)
+        del src, dst
     if a.workload == "batched":
         run_batched(a)
+    elif a.workload == "pore":
+        run_pore(a)
+    elif a.workload == "dpd":
+        run_dpd(a)
     else:
         run_slab(a)
 

@@ -500,8 +500,10 @@ static void density_core(int n, const double *x, const int *wrapk, const nlist_t
 
 static void force_core(int style, int n, const double *x, const double *v, const int *wrapk,
                        const nlist_t *nl, const double *rho, const uint32_t *sig,
-                       const int *type, const orc_params *p, double *f, double *e, double *vir)
+                       const int *type, const orc_params *p, double *f, double *e, double *vir,
+                       const int *mask, const double *phi, const orc_arb *arb)
 {
+    const int wall_scaling = style == 0 && arb && arb->pair_on && mask && phi;
     double L[3] = {p->hi[0] - p->lo[0], p->hi[1] - p->lo[1], p->hi[2] - p->lo[2]};
     const double dtinvsqrt = 1.0 / sqrt(p->dt);
 #pragma omp parallel for schedule(static)
@@ -530,9 +532,16 @@ static void force_core(int style, int n, const double *x, const double *v, const
                 if (wcr < 0) wcr = 0;
                 double wr = wc;
                 double rr = rho[i] + rho[j];
+                double gam = p->gamma[ti][tj], sgm = p->sigma[ti][tj];
+                if (wall_scaling && (mask[i] & arb->pair_mobile_bit) && (mask[j] & arb->pair_wall_bit)) {
+                    /* pair_mdpd_meso.cu:169-185: phi of the mobile particle i */
+                    double ratio = orc_wall_ratio(phi[i], cut);
+                    sgm *= sqrt(ratio);
+                    gam *= ratio;
+                }
                 fpair = (p->A[ti][tj] * wc + p->B[ti][tj] * rr * wcr)
-                        - (p->gamma[ti][tj] * wr * wr * dot * rinv)
-                        + (p->sigma[ti][tj] * wr * rn * dtinvsqrt);
+                        - (gam * wr * wr * dot * rinv)
+                        + (sgm * wr * rn * dtinvsqrt);
                 fpair *= rinv;
                 en += 0.5 * p->A[ti][tj] * cut * wr * wr + 0.5 * p->B[ti][tj] * p->cut_r[ti][tj] * rr * wcr * wcr;
             } else {
@@ -578,13 +587,14 @@ void orc_density(int n, const float *x, const int *type, const int *mask,
 
 static void force_entry(int style, int n, const float *x, const float *v, const double *rho,
                         const uint32_t *sig, const int *type, const orc_params *p,
-                        double *f, double *e, double *vir)
+                        double *f, double *e, double *vir,
+                        const int *mask, const double *phi, const orc_arb *arb)
 {
     double *xd = f2d(3 * n, x), *vd = f2d(3 * n, v);
     int *wk = (int *)malloc(sizeof(int) * 3 * (size_t)(n > 0 ? n : 1));
     nlist_t nl = {0};
     nlist_build(n, xd, p->lo, p->hi, max_cut(p), &nl, wk);
-    force_core(style, n, xd, vd, wk, &nl, rho, sig, type, p, f, e, vir);
+    force_core(style, n, xd, vd, wk, &nl, rho, sig, type, p, f, e, vir, mask, phi, arb);
     nlist_free(&nl);
     free(wk); free(xd); free(vd);
 }
@@ -593,13 +603,97 @@ void orc_force_mdpd(int n, const float *x, const float *v, const double *rho,
                     const uint32_t *sig, const int *type, const orc_params *p,
                     double *f, double *e, double *vir)
 {
-    force_entry(0, n, x, v, rho, sig, type, p, f, e, vir);
+    force_entry(0, n, x, v, rho, sig, type, p, f, e, vir, NULL, NULL, NULL);
+}
+
+void orc_force_mdpd_arb(int n, const float *x, const float *v, const double *rho,
+                        const uint32_t *sig, const int *type, const int *mask,
+                        const double *phi, const orc_params *p, const orc_arb *arb,
+                        double *f, double *e, double *vir)
+{
+    force_entry(0, n, x, v, rho, sig, type, p, f, e, vir, mask, phi, arb);
 }
 
 void orc_force_dpd(int n, const float *x, const float *v, const uint32_t *sig,
                    const int *type, const orc_params *p, double *f, double *e, double *vir)
 {
-    force_entry(1, n, x, v, NULL, sig, type, p, f, e, vir);
+    force_entry(1, n, x, v, NULL, sig, type, p, f, e, vir, NULL, NULL, NULL);
+}
+
+/* ======================================================================
+ * Arbitrary boundary          fix_aboundary_rho_meso.cu:93-94,220-287,
+ *                             pair_mdpd_meso.cu:169-185
+ * ====================================================================== */
+
+double orc_wall_ratio(double phi, double rcw)
+{
+    double ph = phi < 0.5 ? phi : 0.5;                          /* MIN(phi_i, 0.5) */
+    double h = 1.0 - pow(2.088 * ph * ph * ph + 1.478 * ph, 0.25);
+    if (h < 0.025) h = 0.025;
+    h *= rcw;
+    double q = 1.0 - h / rcw;
+    return 1.0 + 0.187 * (rcw / h - 1.0) - 0.093 * q * q * q;
+}
+
+static void aboundary_core(int n, double *x, double *v, const int *wrapk, const nlist_t *nl,
+                           const int *type, const int *mask, const orc_params *p, const orc_arb *arb,
+                           double *phi, double *nw_out)
+{
+    double L[3] = {p->hi[0] - p->lo[0], p->hi[1] - p->lo[1], p->hi[2] - p->lo[2]};
+    const double c = arb->cut_phi;
+    const double c7 = c * c * c * c * c * c * c;
+    const double phi_factor = 105.0 / (16.0 * M_PI * arb->rho_wall * c7);   /* :93 */
+    const double dw_factor = -315.0 / (4.0 * M_PI * arb->rho_wall * c7);    /* :94 */
+    double *nw = (double *)calloc(3 * (size_t)(n > 0 ? n : 1), sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        phi[i] = 0.0;
+        if (!(mask[i] & arb->fix_mobile_bit)) continue;
+        double s = 0.0;
+        for (int k = nl->start[i]; k < nl->start[i + 1]; ++k) {
+            int j = nl->j[k];
+            if (!(mask[j] & arb->fix_wall_bit)) continue;
+            double d[3];
+            pair_dx(x, wrapk, L, i, j, nl->code[k], d);
+            double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (!(rsq < c * c)) continue;
+            double rinv = 1.0 / sqrt(rsq);
+            double r = rsq * rinv;
+            if (r < 1e-10) r = 1e-10;                            /* MAX(r, EPSILON) */
+            double dd = c - r;
+            s += (c + 3.0 * r) * dd * dd * dd * phi_factor;      /* :260 */
+            for (int a = 0; a < 3; ++a) nw[3 * i + a] -= (r * dd * dd * dw_factor) * d[a] * rinv;   /* :261 */
+        }
+        phi[i] = s;
+    }
+    /* bounce-back (:266-283), after every phi is known (walls do not move) */
+    for (int i = 0; i < n; ++i) {
+        if (!(mask[i] & arb->fix_mobile_bit) || !(phi[i] > arb->phi_c)) continue;
+        double *m = nw + 3 * i;
+        double len = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        double u[3] = {m[0] / len, m[1] / len, m[2] / len};
+        double dtfm = 0.5 * p->dt / p->mass[type[i]];
+        double vn = v[3 * i] * u[0] + v[3 * i + 1] * u[1] + v[3 * i + 2] * u[2];
+        double tmp = vn > 0.0 ? vn : 0.0;
+        for (int a = 0; a < 3; ++a) {
+            double orig = x[3 * i + a] - v[3 * i + a] * dtfm;
+            v[3 * i + a] = -v[3 * i + a] + 2.0 * tmp * u[a];
+            x[3 * i + a] = orig + p->dt * v[3 * i + a];
+        }
+    }
+    if (nw_out) memcpy(nw_out, nw, sizeof(double) * 3 * (size_t)n);
+    free(nw);
+}
+
+void orc_aboundary(int n, double *x, double *v, const int *type, const int *mask,
+                   const orc_params *p, const orc_arb *arb, double *phi, double *nw)
+{
+    int *wk = (int *)malloc(sizeof(int) * 3 * (size_t)(n > 0 ? n : 1));
+    nlist_t nl = {0};
+    nlist_build(n, x, p->lo, p->hi, arb->cut_phi, &nl, wk);
+    aboundary_core(n, x, v, wk, &nl, type, mask, p, arb, phi, nw);
+    nlist_free(&nl);
+    free(wk);
 }
 
 /* ======================================================================
@@ -633,7 +727,7 @@ static void eval_forces(const orc_run_cfg *cfg, int n, const double *x, const do
         for (int i = 0; i < n; ++i)
             sig[i] = orc_sig_dpd(tag[i], st, (float)v[3 * i], (float)v[3 * i + 1], (float)v[3 * i + 2]);
     }
-    force_core(cfg->style, n, x, v, wk, nl, rho, sig, type, p, f, NULL, NULL);
+    force_core(cfg->style, n, x, v, wk, nl, rho, sig, type, p, f, NULL, NULL, mask, cfg->phi, cfg->arb);
     if (cfg->addforce_groupbit) {
         for (int i = 0; i < n; ++i)
             if (mask[i] & cfg->addforce_groupbit)
@@ -675,6 +769,10 @@ int orc_run(int n, double *x, double *v, double *f, double *rho, const int *tag,
                 x[3 * i + d] += p->dt * v[3 * i + d];
             }
         }
+        /* post_integrate: wall fraction and bounce-back with the list of the
+         * last build (fix_aboundary_rho_meso.cu:290-317) */
+        if (cfg->arb && cfg->arb->fix_on && cfg->phi)
+            aboundary_core(n, x, v, wk, &nl, type, mask, p, cfg->arb, cfg->phi, NULL);
         /* decide(): every 1 delay 0 check yes (src/neighbor.cpp:1216-1290) */
         int flag = 0;
         for (int i = 0; i < n && !flag; ++i) {

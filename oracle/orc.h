@@ -135,6 +135,38 @@ void orc_force_dpd(int n, const float *x, const float *v, const uint32_t *sig,
                    const int *type, const orc_params *p,
                    double *f, double *e, double *vir);
 
+/* ---------------- arbitrary boundary (aboundary/rho/meso wall branch) ---------------- */
+
+typedef struct {
+    /* pair_style mdpd/meso T rc seed <mobile group> <wall group>:
+     * wall-scaled gamma / sigma for mobile-wall pairs; pair_mdpd_meso.cu:169-185 */
+    int    pair_on, pair_mobile_bit, pair_wall_bit;
+    /* fix aboundary/rho/meso ... cut_phi c phi_c p rho_wall w wall G mobile G:
+     * wall fraction and bounce-back after initial integrate;
+     * fix_aboundary_rho_meso.cu:93-94,220-287 */
+    int    fix_on, fix_mobile_bit, fix_wall_bit;
+    double cut_phi, phi_c, rho_wall;
+} orc_arb;
+
+/* ratio scaling gamma (x ratio) and sigma (x sqrt(ratio)) of a mobile-wall
+ * pair from the mobile particle's wall fraction phi and the pair cutoff rcw;
+ * pair_mdpd_meso.cu:171-182 */
+double orc_wall_ratio(double phi, double rcw);
+/* wall fraction phi_i and unit wall normal of every particle in the fix's
+ * mobile group (0 / zero vector for others), from the wall-group particles j
+ * within cut_phi (all pairs, minimum image); then bounce-back of the mobile
+ * particles with phi_i > phi_c: v <- -v + 2 max(0, v.n) n,
+ * x <- (x - v dtf/m) + dt v_new (fix_aboundary_rho_meso.cu:269-285, with its
+ * dtf/m).  x, v FP64 in/out (3n); phi (n) and nw (3n, may be NULL) out. */
+void orc_aboundary(int n, double *x, double *v, const int *type, const int *mask,
+                   const orc_params *p, const orc_arb *arb, double *phi, double *nw);
+/* mDPD force with the wall scaling (phi of each particle, masks); otherwise
+ * as orc_force_mdpd */
+void orc_force_mdpd_arb(int n, const float *x, const float *v, const double *rho,
+                        const uint32_t *sig, const int *type, const int *mask,
+                        const double *phi, const orc_params *p, const orc_arb *arb,
+                        double *f, double *e, double *vir);
+
 /* ---------------- step loop (FP64 state) ---------------- */
 
 typedef struct {
@@ -147,11 +179,14 @@ typedef struct {
     int      rho_groupbit, nve_groupbit, addforce_groupbit;
     double   addforce[3];
     int      nthreads;       /* OpenMP threads (0 = runtime default) */
+    const orc_arb *arb;      /* arbitrary boundary (NULL: off) */
+    double  *phi;            /* n, wall fractions carried between steps (needed with arb) */
 } orc_run_cfg;
 
 /* ModifiedVerlet::setup + run(nsteps) with neigh_modify every 1 delay 0
  * check yes; mvv_meso.cu:203-283,309-572, fix_nve_meso.cu:63-95,157-178,
- * fix_addforce_meso.cu:72-108.  x, v (3n) are FP64 in/out, f out; the whole
+ * fix_addforce_meso.cu:72-108; with cfg->arb the wall branch runs after
+ * initial integrate (post_integrate) and scales the force.  x, v (3n) are FP64 in/out, f out; the whole
  * state and all arithmetic are FP64 (the DPD signature reads the FP32
  * rounding of v, as dp2sp_merged does).  Returns the number of neighbor
  * builds including the setup build. */

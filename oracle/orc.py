@@ -45,7 +45,13 @@ class RunCfg(ctypes.Structure):
                 ("skin", ctypes.c_double), ("cut_rho", ctypes.c_double), ("lucy", ctypes.c_int),
                 ("rho_groupbit", ctypes.c_int), ("nve_groupbit", ctypes.c_int),
                 ("addforce_groupbit", ctypes.c_int), ("addforce", ctypes.c_double * 3),
-                ("nthreads", ctypes.c_int)]
+                ("nthreads", ctypes.c_int), ("arb", ctypes.c_void_p), ("phi", ctypes.c_void_p)]
+
+
+class Arb(ctypes.Structure):
+    _fields_ = [("pair_on", ctypes.c_int), ("pair_mobile_bit", ctypes.c_int), ("pair_wall_bit", ctypes.c_int),
+                ("fix_on", ctypes.c_int), ("fix_mobile_bit", ctypes.c_int), ("fix_wall_bit", ctypes.c_int),
+                ("cut_phi", ctypes.c_double), ("phi_c", ctypes.c_double), ("rho_wall", ctypes.c_double)]
 
 
 class RanMars(ctypes.Structure):
@@ -77,6 +83,9 @@ def _declare(L):
         "orc_density": (None, [I, P, P, P, P, D, I, I, P]),
         "orc_force_mdpd": (None, [I, P, P, P, P, P, P, P, P, P]),
         "orc_force_dpd": (None, [I, P, P, P, P, P, P, P, P]),
+        "orc_wall_ratio": (D, [D, D]),
+        "orc_aboundary": (None, [I, P, P, P, P, P, P, P, P]),
+        "orc_force_mdpd_arb": (None, [I, P, P, P, P, P, P, P, P, P, P, P, P]),
         "orc_run": (I, [I, P, P, P, P, P, P, P, P, P, I]),
         "orc_ranmars_init": (None, [P, I]),
         "orc_ranmars_uniform": (D, [P]),
@@ -235,9 +244,52 @@ def force_dpd(x, v, sig, types0, p):
     return f, e, vir
 
 
+def arb(mobile_bit, wall_bit, cut_phi=1.0, phi_c=0.5, rho_wall=8.0, pair=True, fix=True):
+    """Arbitrary-boundary settings: the same mobile / wall groups for the pair
+    style's wall scaling and for the fix's wall fraction + bounce-back."""
+    a = Arb()
+    a.pair_on, a.pair_mobile_bit, a.pair_wall_bit = int(pair), mobile_bit, wall_bit
+    a.fix_on, a.fix_mobile_bit, a.fix_wall_bit = int(fix), mobile_bit, wall_bit
+    a.cut_phi, a.phi_c, a.rho_wall = cut_phi, phi_c, rho_wall
+    return a
+
+
+def wall_ratio(phi, rcw=1.0):
+    return lib().orc_wall_ratio(float(phi), float(rcw))
+
+
+def aboundary(x, v, types0, mask, p, a):
+    """wall fraction + bounce-back of the mobile group; returns (x, v, phi, nw)."""
+    x = np.array(x, dtype=np.float64).reshape(-1, 3)
+    v = np.array(v, dtype=np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    t = np.ascontiguousarray(types0, dtype=np.int32)
+    m = np.ascontiguousarray(mask, dtype=np.int32)
+    phi = np.empty(n); nw = np.empty((n, 3))
+    lib().orc_aboundary(n, _p(x), _p(v), _p(t), _p(m), ctypes.byref(p), ctypes.byref(a), _p(phi), _p(nw))
+    return x, v, phi, nw
+
+
+def force_mdpd_arb(x, v, rho, sig, types0, mask, phi, p, a):
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+    v = np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3)
+    n = x.shape[0]
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    sig = np.ascontiguousarray(sig, dtype=np.uint32)
+    t = np.ascontiguousarray(types0, dtype=np.int32)
+    m = np.ascontiguousarray(mask, dtype=np.int32)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    f = np.empty((n, 3)); e = np.empty(n); vir = np.empty((n, 6))
+    lib().orc_force_mdpd_arb(n, _p(x), _p(v), _p(rho), _p(sig), _p(t), _p(m), _p(phi), ctypes.byref(p),
+                             ctypes.byref(a), _p(f), _p(e), _p(vir))
+    return f, e, vir
+
+
 def run(x, v, tag, types0, mask, p, nsteps, style=0, seed=1, ntimestep=0, skin=0.3, cut_rho=0.75,
-        lucy=False, rho_groupbit=1, nve_groupbit=1, addforce_groupbit=0, addforce=(0, 0, 0), nthreads=0):
-    """FP64 setup + run; returns (x, v, f, rho, nbuild)."""
+        lucy=False, rho_groupbit=1, nve_groupbit=1, addforce_groupbit=0, addforce=(0, 0, 0), nthreads=0,
+        arb=None, phi=None):
+    """FP64 setup + run; returns (x, v, f, rho, nbuild).  With arb (see arb()),
+    the wall branch runs each step; phi (n, in/out) carries the wall fractions."""
     x = np.array(x, dtype=np.float64).reshape(-1, 3)
     v = np.array(v, dtype=np.float64).reshape(-1, 3)
     n = x.shape[0]
@@ -249,6 +301,11 @@ def run(x, v, tag, types0, mask, p, nsteps, style=0, seed=1, ntimestep=0, skin=0
     for d in range(3):
         cfg.addforce[d] = addforce[d]
     cfg.nthreads = nthreads
+    if arb is not None:
+        if phi is None or phi.dtype != np.float64 or phi.shape != (n,):
+            raise ValueError("arb needs phi: float64 array of n")
+        cfg.arb = ctypes.cast(ctypes.pointer(arb), ctypes.c_void_p)
+        cfg.phi = phi.ctypes.data_as(ctypes.c_void_p)
     tag = np.ascontiguousarray(tag, dtype=np.int32)
     t = np.ascontiguousarray(types0, dtype=np.int32)
     m = np.ascontiguousarray(mask, dtype=np.int32)

@@ -179,3 +179,57 @@ def test_pool_queue_and_errors_without_a_device(engine):
         p.tune("split", 3)
     with pytest.raises(engine.MdpdError):
         p.stat("nonsense")
+
+
+def test_read_data_atoms_masses_velocities(engine, tmp_path):
+    # LAMMPS data file layout (src/read_data.cpp; Atoms lines of
+    # atom_style mdpd/atomic/meso: id type x y z [ix iy iz])
+    f = tmp_path / "in.data"
+    f.write_text("""LAMMPS data file via write_data
+
+4 atoms
+2 atom types
+
+-3.0 3.0 xlo xhi
+-1.0 1.0 ylo yhi
+0.0 2.0 zlo zhi
+
+Masses
+
+1 1.0
+2 2.5
+
+Atoms # mdpd/atomic/meso
+
+10 2 -2.5 0.5 1.0
+3 1 0.0 0.0 0.5 0 0 1
+7 1 1.5 -0.5 1.5
+1 2 2.9 0.9 0.1
+
+Velocities
+
+3 0.5 0.0 0.0
+10 -1.0 2.0 3.0
+1 0 0 0
+7 0.1 0.2 0.3
+""")
+    s = engine.System()
+    s.command("atom_style mdpd/atomic/meso 3")
+    s.command(f"read_data {f}")
+    assert s.natoms == 4
+    assert list(s.gather("tag")) == [10, 3, 7, 1]
+    assert list(s.gather("type")) == [2, 1, 1, 2]
+    np.testing.assert_array_equal(s.gather("x")[1], [0.0, 0.0, 0.5])
+    np.testing.assert_array_equal(s.gather("v")[0], [-1.0, 2.0, 3.0])
+    s.commands("group heavy type 2\ngroup both union heavy all\nvelocity heavy set 0 NULL 0")
+    np.testing.assert_array_equal(s.gather("v")[0], [0.0, 2.0, 0.0])
+    assert s.group_bit("both") != 0
+    # the box is defined by the file: a second read_data is refused
+    with pytest.raises(engine.MdpdError) as e:
+        s.command(f"read_data {f}")
+    assert e.value.code == 2
+    t = engine.System()
+    t.command("atom_style mdpd/atomic/meso 3")
+    with pytest.raises(engine.MdpdError) as e:
+        t.command(f"read_data {tmp_path / 'missing.data'}")
+    assert e.value.code == 1

@@ -335,3 +335,104 @@ def test_lammps_dpd_statistics(orc):
     assert out["T"] == pytest.approx(1.0, abs=0.05)
     alpha = (out["P"] - 3.0 * out["T"]) / (25.0 * 9.0)
     assert 0.085 < alpha < 0.1047
+
+
+# ---------------------------------------------------------------- arbitrary boundary
+
+RHO_W, CPHI = 8.0, 1.0
+
+
+def _lucy3d(r, c=CPHI, rho_w=RHO_W):
+    # 3-D Lucy kernel normalized over the sphere of radius c, divided by the
+    # wall density: integral over a filled sphere at density rho_w is 1
+    return 105.0 / (16.0 * np.pi * c**7 * rho_w) * (c + 3 * r) * (c - r) ** 3
+
+
+def test_wall_ratio_closed_forms(orc):
+    # phi = 0: h = rcw, no scaling
+    assert orc.wall_ratio(0.0, 1.0) == pytest.approx(1.0, abs=1e-15)
+    # 2.088/8 + 1.478/2 = 1 exactly: h = 0 at phi = 1/2, clamped to 0.025 rcw
+    top = 1 + 0.187 * (1 / 0.025 - 1) - 0.093 * 0.975**3
+    for phi in (0.5, 0.7, 3.0):
+        assert orc.wall_ratio(phi, 1.0) == pytest.approx(top, rel=1e-14)
+    # phi = 1/4 by hand: (2.088/64 + 1.478/4)^(1/4) = 0.402125^(1/4)
+    h = 1 - 0.402125 ** 0.25
+    assert orc.wall_ratio(0.25, 1.0) == pytest.approx(1 + 0.187 * (1 / h - 1) - 0.093 * (1 - h) ** 3, rel=1e-12)
+    # scale invariance in the cutoff and monotone growth towards the wall
+    assert orc.wall_ratio(0.25, 0.8) == pytest.approx(orc.wall_ratio(0.25, 1.0), rel=1e-12)
+    # (h reaches the 0.025 clamp just below phi = 1/2: flat from there)
+    r = [orc.wall_ratio(p, 1.0) for p in np.linspace(0, 0.46, 47)]
+    assert np.all(np.diff(r) > 0)
+    assert orc.wall_ratio(0.49, 1.0) == pytest.approx(top, rel=1e-14)
+
+
+def _arb_state(orc, x, types, L):
+    n = len(x)
+    mask = np.where(types == 2, 1 | 4, 1 | 2)          # all=1, fluid=2, wall=4
+    p = orc.params((0, 0, 0), L, 0.01, mass=(1.0, 1.0), ntypes=2, cut=1.0, cut_r=0.75, A=-40.0, B=25.0,
+                   gamma=18.0, sigma=6.0)
+    a = orc.arb(2, 4, cut_phi=CPHI, phi_c=0.5, rho_wall=RHO_W)
+    return mask, p, a
+
+
+def test_wall_fraction_single_wall_particle(orc):
+    L = (10.0, 10.0, 10.0)
+    for r in (0.1, 0.37, 0.8, 0.999, 1.2):
+        x = np.array([[5.0, 5.0, 5.0], [5.0 + r * 0.6, 5.0 - r * 0.8, 5.0]])
+        types = np.array([0, 1])                           # 0-based: fluid, wall
+        mask, p, a = _arb_state(orc, x, types + 1, L)
+        v = np.zeros((2, 3))
+        xo, vo, phi, nw = orc.aboundary(x, v, types, mask, p, a)
+        want = _lucy3d(r) if r < CPHI else 0.0
+        assert phi[0] == pytest.approx(want, rel=1e-12, abs=1e-300)
+        assert phi[1] == 0.0                                # walls are not mobile
+        if r < CPHI:
+            u = nw[0] / np.linalg.norm(nw[0])
+            assert np.allclose(u, [-0.6, 0.8, 0.0], atol=1e-12)   # from the wall towards the particle
+        assert np.array_equal(xo, x) and np.array_equal(vo, v)   # phi < phi_c: no bounce
+
+
+def test_wall_fraction_of_a_half_space(orc):
+    # walls fill the slab z in [8, 12) as a simple cubic lattice of spacing
+    # 1/2 (density 8, one lattice point per cell of the slab); probes on the
+    # slab face z = 8 see half of the kernel (phi -> 1/2), probes 2 deep see
+    # all of it (phi -> 1), up to the lattice-sum error of a smooth kernel
+    rng = np.random.default_rng(5)
+    Lx, Lz = 12.0, 12.0
+    g = np.arange(24) * 0.5 + 0.25
+    walls = np.array([[a, b, c] for a in g for b in g for c in 8.25 + np.arange(8) * 0.5])
+    nw_ = len(walls)
+    probes = np.c_[rng.uniform(0, Lx, (400, 2)), np.full(400, 8.0)]
+    deep = np.c_[rng.uniform(0, Lx, (400, 2)), np.full(400, 10.0)]
+    x = np.r_[walls, probes, deep]
+    types = np.r_[np.ones(nw_, int), np.zeros(800, int)]
+    mask, p, a = _arb_state(orc, x, types + 1, (Lx, Lx, Lz))
+    a.phi_c = 10.0                                          # no bounce: phi only
+    _, _, phi, nw = orc.aboundary(x, np.zeros_like(x), types, mask, p, a)
+    surf, inner = phi[nw_:nw_ + 400], phi[nw_ + 400:]
+    assert abs(surf.mean() - 0.5) < 0.005
+    assert abs(inner.mean() - 1.0) < 0.01
+    # the normal at the surface points out of the wall (-z here)
+    assert np.mean(nw[nw_:nw_ + 400, 2] < 0) > 0.95
+
+
+def test_bounce_back(orc):
+    # a fluid particle 0.1 inside the face (z = 5) of a three-layer lattice
+    # wall, centred over a lattice point: phi > 1/2 and the normal is +z
+    g = np.arange(-3, 4) * 0.5
+    layer = np.array([[5 + gx, 5 + gy] for gx in g for gy in g])
+    walls = np.concatenate([np.c_[layer, np.full(len(layer), z)] for z in (4.75, 4.25, 3.75)])
+    for v0, want in (((0.0, 0.0, -1.0), (0.0, 0.0, 1.0)),      # into the wall: reversed
+                     ((1.0, 0.5, 1.0), (-1.0, -0.5, 1.0))):   # leaving: tangential reversed
+        x = np.r_[[[5.0, 5.0, 4.9]], walls]
+        v = np.zeros_like(x)
+        v[0] = v0
+        types = np.r_[0, np.ones(len(walls), int)]
+        mask, p, a = _arb_state(orc, x, types + 1, (10.0, 10.0, 10.0))
+        xo, vo, phi, nw = orc.aboundary(x, v, types, mask, p, a)
+        assert phi[0] > 0.5
+        assert np.allclose(nw[0] / np.linalg.norm(nw[0]), [0, 0, 1], atol=1e-12)
+        assert np.allclose(vo[0], want, atol=1e-12)
+        # x <- (x - v dtf/m) + dt v_new, with the reference's dtf = dt/2
+        assert np.allclose(xo[0], x[0] - np.array(v0) * 0.005 + 0.01 * np.array(want), atol=1e-12)
+        assert np.array_equal(xo[1:], x[1:])

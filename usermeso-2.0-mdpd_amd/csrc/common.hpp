@@ -71,6 +71,12 @@ struct SysDev {
     int job;                     // pooled engine: caller's job id of this slot
     int nne;                     // non-empty cells at the last re-sort
     int split_last;              // pooled engine: waves per tile of the system's last pass
+    // arbitrary boundary: wall-scaled gamma / sigma of mobile-wall pairs
+    // (pair_style mdpd/meso ... mobile wall) and wall fraction + bounce-back
+    // (fix aboundary/rho/meso ... wall G mobile G)
+    int pair_arb, pair_mobile_bit, pair_wall_bit;
+    int fix_arb, fix_mobile_bit, fix_wall_bit;
+    float cut_phi, phi_c, phi_factor, dw_factor;
     // pooled engine: Neighbor::decide on the device (src/neighbor.cpp:1216-1290)
     int ago;                     // steps since the last build
     int n_every, n_delay, n_check;
@@ -83,6 +89,7 @@ struct PoolDev {
     int *tag[2], *mask[2], *origin[2], *image[2];
     float4 *xhold;       // positions at the last build
     float *rho;
+    float *phi[2];       // wall fraction of the step (flips with tag / mask at re-sort)
     float4 *frc;         // fx, fy, fz, e_pair   (output steps only)
     float *vir;          // 6 * cap, SoA blocks  (output steps only)
     float4 *vend;        // end-of-step velocity  (output steps only)
@@ -111,7 +118,8 @@ struct PoolDev {
     float *coef;         // pooled coefficient tables
     float *minv;         // pooled per-type 1/m
     float *mass;         // pooled per-type m
-    int cap;             // particle slots (multiple of 64)
+    int cap;             // particle slots (multiple of 64): array extent and strides
+    int nact;            // slots [0, nact) hold systems: extent of the launches (<= cap)
     int ncellslots;
     int nsys;
     int maxnbr;
@@ -237,6 +245,20 @@ __device__ inline float pair_gaussian(uint32_t si, uint32_t sj)
     const float f = (v0 & 0x80000000u) ? -s : s;   // times +1 / -1 by the top bit
     const float r = __builtin_amdgcn_sqrtf(-2.0f * ln_u32(v1 > 1u ? v1 : 1u));
     return fminf(fmaxf(r * f, -4.0f), 4.0f);
+}
+
+// gamma / sigma scale of a mobile-wall pair from the mobile particle's wall
+// fraction phi and the pair cutoff rcw (pair_mdpd_meso.cu:171-182):
+// h = max(1 - (2.088 phi^3 + 1.478 phi)^(1/4), 0.025) rcw with phi <= 1/2,
+// ratio = 1 + 0.187 (rcw/h - 1) - 0.093 (1 - h/rcw)^3
+__device__ inline float wall_ratio(float phi, float rcw)
+{
+    const float ph = fminf(phi, 0.5f);
+    const float q4 = 2.088f * ph * ph * ph + 1.478f * ph;
+    float h = 1.0f - __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(q4));
+    h = fmaxf(h, 0.025f) * rcw;
+    const float q = 1.0f - h / rcw;
+    return 1.0f + 0.187f * (rcw / h - 1.0f) - 0.093f * q * q * q;
 }
 
 // image code -> shift multipliers

@@ -33,6 +33,9 @@ hipError_t launch_force(const PoolDev &P, int style, int mode, bool uni, int spl
 // the same over the 64-particle tiles [tile0, tile0 + ntiles) only
 hipError_t launch_force_range(const PoolDev &P, int style, int mode, bool uni, int split, int tile0, int ntiles,
                               hipStream_t st);
+// wall branch of aboundary/rho/meso (wall fraction, bounce-back, displacement
+// flags) on the buffer a pass just wrote; pooled: only systems that integrated
+hipError_t launch_aboundary(const PoolDev &P, int pooled, hipStream_t st);
 // pooled engine, finished system s: current data into buffer 0 (sel = sela = 0)
 hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, hipStream_t st);
 

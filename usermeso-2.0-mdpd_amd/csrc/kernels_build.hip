@@ -23,7 +23,7 @@ namespace mdpd {
 __global__ void __launch_bounds__(256) k_keys(PoolDev P)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.cap) return;
+    if (i >= P.nact) return;
     const int s = P.wave_sys[i >> 6];
     if (s < 0) return;
     SysDev *d = P.sys + s;
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256) k_relist_cells(PoolDev P)
 __global__ void __launch_bounds__(256) k_scatter(PoolDev P)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.cap) return;
+    if (i >= P.nact) return;
     const int s = P.wave_sys[i >> 6];
     if (s < 0) return;
     SysDev *d = P.sys + s;
@@ -265,6 +265,7 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
             P.mask[cura ^ 1][dst] = P.mask[cura][idx];
             P.origin[cura ^ 1][dst] = P.origin[cura][idx];
             P.image[cura ^ 1][dst] = P.image[cura][idx];
+            P.phi[cura ^ 1][dst] = P.phi[cura][idx];
             if (P.key) P.key[(size_t)P.cap + dst] = P.key[idx];
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -363,11 +364,17 @@ __global__ void __launch_bounds__(WAVE * K5_WAVES) k_nlist(PoolDev P, int relist
                 const float4 q = pos[start + c0 + lane];
                 myx = q.x; myy = q.y; myz = q.z;
             }
-            for (int b0 = 0; b0 < total; b0 += WAVE) {
+            // lanes take the flattened candidates of the 27 stencil cells, 64
+            // at a time; the next batch's index, image code and position are
+            // fetched before the current batch is tested, so the gather's
+            // latency overlaps the centre loop
+            auto fetch = [&](int b0, int &j, int &code, int &k, float4 &q) {
                 const int mi = b0 + lane;
-                const bool valid = mi < total;
-                int k = 0;
-                if (valid) {
+                j = 0;
+                code = IMG_ZERO;
+                k = 0;
+                q = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (mi < total) {
                     // largest k with pre[k] <= mi (pre is non-decreasing)
                     int lo = 0, hi = 26;
                     while (lo < hi) {
@@ -375,13 +382,21 @@ __global__ void __launch_bounds__(WAVE * K5_WAVES) k_nlist(PoolDev P, int relist
                         if (s_pre[mid] <= mi) lo = mid; else hi = mid - 1;
                     }
                     k = lo;
-                }
-                int j = 0, code = IMG_ZERO;
-                float xj = 0.f, yj = 0.f, zj = 0.f;
-                if (valid) {
                     j = s_start[k] + (mi - s_pre[k]);
                     code = s_code[k];
-                    const float4 pj = pos[j];
+                    q = pos[j];
+                }
+            };
+            int jn, cn, kn;
+            float4 qn;
+            fetch(0, jn, cn, kn, qn);
+            for (int b0 = 0; b0 < total; b0 += WAVE) {
+                const bool valid = b0 + lane < total;
+                const int j = jn, code = cn, k = kn;
+                const float4 pj = qn;
+                if (b0 + WAVE < total) fetch(b0 + WAVE, jn, cn, kn, qn);
+                float xj = 0.f, yj = 0.f, zj = 0.f;
+                if (valid) {
                     xj = pj.x + s_sh[k * 3 + 0];
                     yj = pj.y + s_sh[k * 3 + 1];
                     zj = pj.z + s_sh[k * 3 + 2];
@@ -454,7 +469,7 @@ __global__ void k_commit_rebuild(PoolDev P)
 __global__ void __launch_bounds__(256) k_signatures(PoolDev P)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.cap) return;
+    if (i >= P.nact) return;
     const int s = P.wave_sys[i >> 6];
     if (s < 0) return;
     SysDev *d = P.sys + s;
@@ -476,15 +491,15 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 constexpr int CELL_GRID_MAX = 2048;
 static inline int cell_grid(const PoolDev &P, int waves_per_block)
 {
-    return std::max(1, std::min(nblk(P.cap, waves_per_block * 8), CELL_GRID_MAX));
+    return std::max(1, std::min(nblk(P.nact, waves_per_block * 8), CELL_GRID_MAX));
 }
 
 hipError_t launch_rebuild(const PoolDev &P, hipStream_t st)
 {
-    if (P.cap == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_keys, dim3(nblk(P.cap, 256)), dim3(256), 0, st, P);
+    if (P.nact == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_keys, dim3(nblk(P.nact, 256)), dim3(256), 0, st, P);
     hipLaunchKernelGGL(k_scan, dim3(P.nsys), dim3(1024), 0, st, P);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(P.cap, 256)), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(P.nact, 256)), dim3(256), 0, st, P);
     hipLaunchKernelGGL(k_sortgather, dim3(cell_grid(P, 4)), dim3(256), 0, st, P);
     hipLaunchKernelGGL(k_nlist, dim3(cell_grid(P, K5_WAVES)), dim3(WAVE * K5_WAVES), nlist_lds_bytes(P.maxnbr), st, P, 0);
     return hipGetLastError();
@@ -512,7 +527,7 @@ hipError_t launch_commit_rebuild(const PoolDev &P, hipStream_t st)
 
 hipError_t launch_signatures(const PoolDev &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_signatures, dim3(nblk(P.cap, 256)), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(k_signatures, dim3(nblk(P.nact, 256)), dim3(256), 0, st, P);
     return hipGetLastError();
 }
 

@@ -293,16 +293,30 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
     const int cur = MODE == MODE_HARVEST ? d->sel ^ 1 : d->sel;
     const float4 *pos = P.pos[cur];
     const float4 *vel = P.vel[cur];
+    const int *maskv = P.mask[d->sela];
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
-    float rho_i = 0.f;
-    int ti = 0, cnt = 0;
+    float rho_i = 0.f, phi_i = 0.f;
+    int ti = 0, cnt = 0, mk = 0;
     if (valid) {
         pi = pos[i];
         vi = vel[i];
         ti = __float_as_int(pi.w);
+        mk = maskv[i];
         if (STYLE == STYLE_MDPD) rho_i = P.rho[i];
         cnt = P.ccount[((size_t)tile * SPLIT + w) * WAVE + lane];
     }
+    // arbitrary boundary: a mobile particle scales gamma / sigma of its pairs
+    // with wall particles by its wall fraction (pair_mdpd_meso.cu:169-185)
+    const bool arb_i = STYLE == STYLE_MDPD && d->pair_arb && (mk & d->pair_mobile_bit);
+    if (arb_i) phi_i = P.phi[d->sela][i];
+    const int wall_bit = d->pair_wall_bit;
+    auto wall_scale = [&](PairC &c, uint32_t j) {
+        if (arb_i && (maskv[j] & wall_bit)) {
+            const float ratio = wall_ratio(phi_i, c.rc);
+            c.gamma *= ratio;
+            c.sigma *= __builtin_amdgcn_sqrtf(ratio);
+        }
+    };
     const float dtis = d->dtinvsqrt;
     const float *coef = P.coef + d->coef_base;
     const int nt = d->ntypes;
@@ -320,8 +334,10 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
         const float4 w0 = vel[j0], w1 = vel[j1];
         const float r0 = STYLE == STYLE_MDPD ? P.rho[j0] : 0.f;
         const float r1 = STYLE == STYLE_MDPD ? P.rho[j1] : 0.f;
-        const PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
-        const PairC c1 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q1.w));
+        PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
+        PairC c1 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q1.w));
+        wall_scale(c0, j0);
+        wall_scale(c1, j1);
         pair_term<STYLE, EV>(a, pi, vi, rho_i, q0, w0, r0, e0 >> IMG_SHIFT, c0, dtis, d);
         pair_term<STYLE, EV>(a, pi, vi, rho_i, q1, w1, r1, e1 >> IMG_SHIFT, c1, dtis, d);
     }
@@ -330,7 +346,8 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
         const uint32_t j0 = e0 & IDX_MASK;
         const float4 q0 = pos[j0];
         const float r0 = STYLE == STYLE_MDPD ? P.rho[j0] : 0.f;
-        const PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
+        PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
+        wall_scale(c0, j0);
         pair_term<STYLE, EV>(a, pi, vi, rho_i, q0, vel[j0], r0, e0 >> IMG_SHIFT, c0, dtis, d);
     }
     if (SPLIT > 1) {
@@ -344,7 +361,6 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
     int moved = 0;
     if (w == 0 && valid) {
         float fx = a.v[0], fy = a.v[1], fz = a.v[2];
-        const int mk = P.mask[d->sela][i];
         if (mk & d->add_bit) { fx += d->add[0]; fy += d->add[1]; fz += d->add[2]; }
         if (EV) {
             P.frc[i] = make_float4(fx, fy, fz, 0.5f * a.v[3]);
@@ -384,6 +400,88 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
         }
     }
     if (MODE == MODE_FUSED || MODE == MODE_FUSED_OUT || MODE == MODE_PHASED) tile_flags(P.moved, tile, 1, moved);
+}
+
+// --------------------------------------------------------------------------
+// Wall branch of aboundary/rho/meso after the initial integrate
+// (FixArbitraryBoundaryRho::post_integrate / gpu_aboundary,
+// fix_aboundary_rho_meso.cu:220-317): wall fraction phi_i and wall normal of
+// each mobile particle from the wall particles within cut_phi (every entry of
+// its neighbor row, positions after the drift), bounce-back when
+// phi_i > phi_c (v <- -v + 2 max(0, v.n) n, x <- x - v dtf/m + dt v_new),
+// then the tile's displacement flag from the final positions.  Works on the
+// buffer the pass just wrote (sel ^ 1, before the step commit); one wave per
+// 64-particle tile.  `pooled`: apply only to systems whose pass integrated.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(WAVE) k_aboundary(PoolDev P, int ntiles, int pooled)
+{
+    const int tile = xcd_logical(blockIdx.x, ntiles);
+    if (tile >= ntiles) return;
+    const int s = P.wave_sys[tile];
+    if (s < 0) return;
+    const SysDev *d = P.sys + s;
+    if (!d->active || !d->fix_arb) return;
+    if (pooled && (d->overflow || d->phase == PH_LAST)) return;
+    const int lane = threadIdx.x;
+    const int i = tile * WAVE + lane;
+    const bool valid = i < d->part_base + d->n;
+    const int sa = d->sela;
+    float4 *pos = P.pos[d->sel ^ 1];
+    float4 *vel = P.vel[d->sel ^ 1];
+    const int *mask = P.mask[sa];
+    int moved = 0;
+    if (valid) {
+        float4 p = pos[i];
+        if (mask[i] & d->fix_mobile_bit) {
+            const int maxnbr = P.maxnbr;
+            const int cnt = min(P.ncount[i], maxnbr);
+            const int *row = P.nlist + (size_t)tile * maxnbr * WAVE + lane;
+            const float c = d->cut_phi, c2 = c * c;
+            float phi = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
+            for (int q = 0; q < cnt; ++q) {
+                const uint32_t e = (uint32_t)row[(size_t)q * WAVE];
+                const uint32_t j = e & IDX_MASK;
+                if (!(mask[j] & d->fix_wall_bit)) continue;
+                const float4 pj = pos[j];
+                float xj = pj.x, yj = pj.y, zj = pj.z;
+                const uint32_t code = e >> IMG_SHIFT;
+                if (code != IMG_ZERO) apply_image(code, d, xj, yj, zj);
+                const float dx = p.x - xj, dy = p.y - yj, dz = p.z - zj;
+                const float rsq = dx * dx + dy * dy + dz * dz;
+                if (!(rsq < c2)) continue;
+                const float rinv = __builtin_amdgcn_rsqf(rsq);
+                const float r = fmaxf(rsq * rinv, 1e-10f);
+                const float dd = c - r;
+                phi += (c + 3.f * r) * dd * dd * dd * d->phi_factor;
+                const float g = r * dd * dd * d->dw_factor * rinv;
+                nx -= g * dx; ny -= g * dy; nz -= g * dz;
+            }
+            P.phi[sa][i] = phi;
+            if (phi > d->phi_c) {
+                const float inv = __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
+                nx *= inv; ny *= inv; nz *= inv;
+                float4 v = vel[i];
+                const float dtfm = d->dtf * P.minv[d->mass_base + __float_as_int(p.w)];
+                const float ox = p.x - v.x * dtfm, oy = p.y - v.y * dtfm, oz = p.z - v.z * dtfm;
+                const float tmp = fmaxf(0.f, v.x * nx + v.y * ny + v.z * nz);
+                v.x = -v.x + 2.f * tmp * nx;
+                v.y = -v.y + 2.f * tmp * ny;
+                v.z = -v.z + 2.f * tmp * nz;
+                p.x = ox + d->dt * v.x;
+                p.y = oy + d->dt * v.y;
+                p.z = oz + d->dt * v.z;
+                if (d->style == STYLE_DPD)      // the DPD signature reads v
+                    v.w = __uint_as_float(sig_dpd(P.tag[sa][i], d->seed_next, v.x, v.y, v.z));
+                pos[i] = p;
+                vel[i] = v;
+            }
+        }
+        const float4 h = P.xhold[i];
+        const float ex = p.x - h.x, ey = p.y - h.y, ez = p.z - h.z;
+        moved = ex * ex + ey * ey + ez * ez > d->trig2;
+    }
+    const uint64_t any = __ballot(moved);
+    if (lane == 0) P.moved[tile] = any != 0 ? 1 : 0;
 }
 
 // --------------------------------------------------------------------------
@@ -460,6 +558,7 @@ __global__ void __launch_bounds__(256) k_harvest(PoolDev P, int s, int pb, int n
         P.mask[0][i] = P.mask[1][i];
         P.origin[0][i] = P.origin[1][i];
         P.image[0][i] = P.image[1][i];
+        P.phi[0][i] = P.phi[1][i];
     }
 }
 
@@ -495,7 +594,7 @@ static void launch_first_t(const PoolDev &P, bool with_rho, int nt, hipStream_t 
 
 hipError_t launch_first(const PoolDev &P, bool with_rho, int split, hipStream_t st)
 {
-    const int nt = P.cap / WAVE;
+    const int nt = P.nact / WAVE;
     if (nt == 0) return hipSuccess;
     switch (split) {
     case 1: launch_first_t<1>(P, with_rho, nt, st); break;
@@ -513,6 +612,14 @@ hipError_t launch_commit_step(const PoolDev &P, int kind, int split, hipStream_t
     return hipGetLastError();
 }
 
+hipError_t launch_aboundary(const PoolDev &P, int pooled, hipStream_t st)
+{
+    const int nt = P.nact / WAVE;
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_aboundary, dim3(xcd_grid(nt)), dim3(WAVE), 0, st, P, nt, pooled);
+    return hipGetLastError();
+}
+
 hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, hipStream_t st)
 {
     if (npad <= 0) return hipSuccess;
@@ -523,7 +630,7 @@ hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, hipS
 
 hipError_t launch_initial(const PoolDev &P, hipStream_t st)
 {
-    const int nl = P.cap / BLK;
+    const int nl = P.nact / BLK;
     if (nl == 0) return hipSuccess;
     hipLaunchKernelGGL(k_initial, dim3(xcd_grid(nl)), dim3(BLK), 0, st, P, nl);
     return hipGetLastError();
@@ -552,7 +659,7 @@ static hipError_t launch_force_s(const PoolDev &P, bool uni, int split, int t0, 
 
 hipError_t launch_force(const PoolDev &P, int style, int mode, bool uni, int split, hipStream_t st)
 {
-    return launch_force_range(P, style, mode, uni, split, 0, P.cap / WAVE, st);
+    return launch_force_range(P, style, mode, uni, split, 0, P.nact / WAVE, st);
 }
 
 hipError_t launch_force_range(const PoolDev &P, int style, int mode, bool uni, int split, int t0, int nt,

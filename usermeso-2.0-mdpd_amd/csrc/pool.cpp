@@ -60,6 +60,13 @@ struct RangeAlloc {
         }
         return -1;
     }
+    // end of the highest range in use (0 when empty)
+    long long high() const
+    {
+        if (fr.empty()) return cap;
+        auto last = std::prev(fr.end());
+        return last->first + last->second == cap ? last->first : cap;
+    }
     void give(long long st, long long len)
     {
         used -= len;
@@ -134,7 +141,7 @@ struct Job {
 
 // bytes of device memory per particle slot (state, scratch, 2 cell slots)
 // and per neighbor-row entry (nlist + clist)
-constexpr double BYTES_PER_SLOT = 256.0;
+constexpr double BYTES_PER_SLOT = 264.0;
 constexpr double BYTES_PER_ENTRY = 8.0;
 constexpr int CELLS_PER_SLOT = 2;
 
@@ -150,9 +157,10 @@ struct mdpd_pool {
     // ---- device arrays (allocated at the first admission) ----
     bool allocated = false;
     int cap = 0, nslots = 0, maxnbr = 0;
+    int nact = 0;                         // launch extent: high-water slot of the resident systems
     DevBuf<float4> pos[2], vel[2], xhold, frc, vend;
     DevBuf<int> tag[2], mask[2], origin[2], image[2];
-    DevBuf<float> rho, vir, coef, minv, mmass;
+    DevBuf<float> rho, vir, coef, minv, mmass, phi[2];
     DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys;
     DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl, wl, work_total;
     DevBuf<SysDev> dsys;
@@ -174,7 +182,7 @@ struct mdpd_pool {
     bool graph_valid = false;
     int passes_per_poll = 8;
     std::set<std::pair<int, int>> combos;  // (style, single type) of admitted jobs
-    bool with_rho = false;
+    bool with_rho = false, with_arb = false;
 
     // ---- status polling and staging ----
     int *pin_status[2] = {nullptr, nullptr};
@@ -259,6 +267,7 @@ struct mdpd_pool {
             P.tag[b] = tag[b].p; P.mask[b] = mask[b].p; P.origin[b] = origin[b].p; P.image[b] = image[b].p;
         }
         P.xhold = xhold.p; P.rho = rho.p; P.frc = frc.p; P.vir = vir.p; P.vend = vend.p;
+        P.phi[0] = phi[0].p; P.phi[1] = phi[1].p;
         P.pcell = pcell.p; P.pslot = pslot.p; P.psub = psub.p; P.tmp = tmp.p; P.key = nullptr;
         P.cell_count = cell_count.p; P.cell_start = cell_start.p; P.cell_xyz = cell_xyz.p;
         P.cell_rank = cell_rank.p; P.cell_sys = cell_sys.p;
@@ -266,7 +275,7 @@ struct mdpd_pool {
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
         P.cl = cl.p; P.wl = wl.p; P.work_total = work_total.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
-        P.cap = cap; P.ncellslots = nslots; P.nsys = max_sys; P.maxnbr = maxnbr;
+        P.cap = cap; P.nact = nact; P.ncellslots = nslots; P.nsys = max_sys; P.maxnbr = maxnbr;
         return P;
     }
 
@@ -346,7 +355,7 @@ struct mdpd_pool {
             tag[b].alloc(cap); mask[b].alloc(cap); origin[b].alloc(cap); image[b].alloc(cap);
         }
         xhold.alloc(cap); frc.alloc(cap); vend.alloc(cap);
-        rho.alloc(cap); vir.alloc((size_t)6 * cap);
+        rho.alloc(cap); vir.alloc((size_t)6 * cap); phi[0].alloc(cap); phi[1].alloc(cap);
         pcell.alloc(cap); pslot.alloc(cap); psub.alloc(cap); tmp.alloc(cap);
         ncount.alloc(cap); ccount.alloc((size_t)4 * cap);
         nlist.alloc((size_t)cap * maxnbr);
@@ -429,6 +438,7 @@ struct mdpd_pool {
             upload(j, s, pb * 256, cb);
             any = true;
         }
+        if (any) update_extent();
         return any;
     }
 
@@ -497,6 +507,7 @@ struct mdpd_pool {
         hip_check(hipMemcpyAsync(minv.p + d.mass_base, ms + nt, sizeof(float) * nt, hipMemcpyHostToDevice, stream),
                   "upload");
         hip_check(hipMemsetAsync(cell_count.p + cbase, 0, sizeof(int) * nc1, stream), "memset");
+        hip_check(hipMemsetAsync(phi[0].p + pbase, 0, sizeof(float) * npad, stream), "memset");
         up(dsys.p + s, 12);
         hipEvent_t e = take_event();
         hip_check(hipEventRecord(e, stream), "hipEventRecord");
@@ -512,10 +523,24 @@ struct mdpd_pool {
         const auto combo = std::make_pair(d.style, j.ntypes == 1 ? 1 : 0);
         if (!combos.count(combo)) { combos.insert(combo); graph_valid = false; }
         if (d.style == STYLE_MDPD && d.rho_bit && !with_rho) { with_rho = true; graph_valid = false; }
+        if (d.fix_arb && !with_arb) { with_arb = true; graph_valid = false; }
         // the host copy of the input is no longer needed
         std::vector<float4>().swap(j.hp); std::vector<float4>().swap(j.hv);
         std::vector<int>().swap(j.ht); std::vector<int>().swap(j.hm);
         std::vector<int>().swap(j.ho); std::vector<int>().swap(j.himg);
+    }
+
+    // launches cover slots [0, nact): the highest resident range, rounded up
+    // to 64K slots so the captured pass is rebuilt only when it moves by that
+    void update_extent()
+    {
+        const long long hw = parts.high() * 256;
+        const long long q = 65536;
+        const int want = (int)std::min<long long>(cap, (hw + q - 1) / q * q);
+        if (want != nact) {
+            nact = want;
+            graph_valid = false;
+        }
     }
 
     void recycle_uploads(bool block)
@@ -565,6 +590,7 @@ struct mdpd_pool {
         hip_check(launch_first(P, with_rho, sp, stream), "first pass");
         mark(PP_FORCE);
         for (auto &c : combos) hip_check(launch_force(P, c.first, MODE_PHASED, c.second != 0, sp, stream), "force");
+        if (with_arb) hip_check(launch_aboundary(P, 1, stream), "aboundary");
         mark(PP_COMMIT);
         hip_check(launch_commit_step(P, 2, sp, stream), "commit step");
         mark(PP_COUNT);
@@ -818,6 +844,7 @@ struct mdpd_pool {
             finished.push_back(j.id);
             ++done;
         }
+        if (done) update_extent();
         return done;
     }
 

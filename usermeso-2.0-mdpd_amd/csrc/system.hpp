@@ -25,6 +25,7 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <fstream>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -117,6 +118,9 @@ struct FixDesc {
     int groupbit = 0;
     double rd = 0;       // aboundary/rho/meso
     int lucy = 0;
+    // aboundary/rho/meso wall branch (fix_aboundary_rho_meso.cu:46-101)
+    double cut_phi = 0, phi_c = 0, rho_wall = 0;
+    int wall_bit = 0, mobile_bit = 0, arb = 0;
     double f[3] = {0, 0, 0};   // addforce/meso
 };
 
@@ -154,6 +158,7 @@ struct mdpd_system {
     double rho_est = 0;
     int pair_style = -1;
     double temperature = 1.0, cut_global = 1.0;
+    int pair_mobile_bit = 0, pair_wall_bit = 0;   // pair_style mdpd/meso ... mobile wall
     int seed = 0;
     int ntypes = 0;
     bool box_set = false;
@@ -187,7 +192,7 @@ struct mdpd_system {
     SysDev hsys{};
     DevBuf<float4> pos[2], vel[2], xhold, frc, vend;
     DevBuf<int> tag[2], mask[2], origin[2], image[2];
-    DevBuf<float> rho, vir, coef, minv, mmass;
+    DevBuf<float> rho, vir, coef, minv, mmass, phi[2];
     DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys;
     DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl, wl, work_total;
     DevBuf<unsigned long long> key;
@@ -321,6 +326,11 @@ struct mdpd_system {
             need(2);
             ntimestep = parse_int(w[1]);
             dirty = true;
+        } else if (c == "read_data") {
+            need(2);
+            read_data(w[1]);
+        } else if (c == "velocity") {
+            cmd_velocity(w);
         } else if (c == "thermo") {
             need(2);
             thermo_every = (int)parse_int(w[1]);
@@ -330,6 +340,126 @@ struct mdpd_system {
         } else {
             fail(MDPD_E_UNSUPPORTED, "unknown or unsupported command: %s", c.c_str());
         }
+    }
+
+    // read_data for atom_style mdpd/atomic/meso or dpd/atomic/meso: header
+    // (atoms, atom types, xlo xhi, ylo yhi, zlo zhi), Masses, Atoms
+    // (id type x y z [ix iy iz]; AtomVecMDPDAtomic::data_atom,
+    // atom_vec_mdpd_atomic_meso.cu:609-637) and Velocities (id vx vy vz);
+    // src/read_data.cpp
+    void read_data(const std::string &path)
+    {
+        if (box_set) fail(MDPD_E_STATE, "Cannot read_data after simulation box is defined");
+        if (atom_style < 0) fail(MDPD_E_STATE, "atom_style must be set before read_data");
+        std::ifstream in(path);
+        if (!in) fail(MDPD_E_ARG, "Cannot open file %s", path.c_str());
+        auto strip = [](std::string l) {
+            l = l.substr(0, l.find('#'));
+            const size_t a = l.find_first_not_of(" \t\r"), b = l.find_last_not_of(" \t\r");
+            return a == std::string::npos ? std::string() : l.substr(a, b - a + 1);
+        };
+        std::string line;
+        std::getline(in, line);                       // title line
+        long long natoms = -1;
+        int nt = -1;
+        double blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+        bool have[3] = {false, false, false};
+        std::string section;
+        // header
+        while (std::getline(in, line)) {
+            const std::string t = strip(line);
+            if (t.empty()) continue;
+            std::istringstream is(t);
+            std::vector<std::string> f;
+            for (std::string x; is >> x;) f.push_back(x);
+            if (f.size() == 2 && f[1] == "atoms") natoms = parse_int(f[0]);
+            else if (f.size() == 3 && f[1] == "atom" && f[2] == "types") nt = (int)parse_int(f[0]);
+            else if (f.size() == 4 && f[2] == "xlo" && f[3] == "xhi") { blo[0] = parse_num(f[0]); bhi[0] = parse_num(f[1]); have[0] = true; }
+            else if (f.size() == 4 && f[2] == "ylo" && f[3] == "yhi") { blo[1] = parse_num(f[0]); bhi[1] = parse_num(f[1]); have[1] = true; }
+            else if (f.size() == 4 && f[2] == "zlo" && f[3] == "zhi") { blo[2] = parse_num(f[0]); bhi[2] = parse_num(f[1]); have[2] = true; }
+            else if (f.size() == 6 && f[3] == "xy") fail(MDPD_E_UNSUPPORTED, "triclinic boxes are outside this engine");
+            else if (f.size() >= 2 && (f[1] == "bonds" || f[1] == "angles" || f[1] == "dihedrals" || f[1] == "impropers")) {
+                if (parse_int(f[0]) != 0) fail(MDPD_E_UNSUPPORTED, "molecular topology is outside this engine");
+            } else if (f.size() >= 3 && f[2] == "types") {
+            } else { section = f[0]; break; }
+        }
+        if (natoms < 0 || nt < 1) fail(MDPD_E_ARG, "Invalid data file header in %s", path.c_str());
+        if (nt > MAXTYPES) fail(MDPD_E_ARG, "number of atom types must be 1..%d", MAXTYPES);
+        for (int d = 0; d < 3; ++d)
+            if (!have[d] || !(bhi[d] > blo[d])) fail(MDPD_E_ARG, "Invalid box bounds in data file %s", path.c_str());
+        ntypes = nt;
+        for (int d = 0; d < 3; ++d) { lo[d] = blo[d]; hi[d] = bhi[d]; }
+        box_set = true;
+        std::vector<double> vx;
+        std::map<int, size_t> where;
+        const size_t first = h_tag.size();
+        while (!section.empty()) {
+            const std::string sec = section;
+            section.clear();
+            long long rows = 0;
+            const long long want = sec == "Masses" ? nt : natoms;
+            if (sec != "Masses" && sec != "Atoms" && sec != "Velocities")
+                fail(MDPD_E_UNSUPPORTED, "read_data section %s is outside this engine", sec.c_str());
+            while (rows < want && std::getline(in, line)) {
+                const std::string t = strip(line);
+                if (t.empty()) continue;
+                std::istringstream is(t);
+                std::vector<std::string> f;
+                for (std::string x; is >> x;) f.push_back(x);
+                if (sec == "Masses") {
+                    if (f.size() < 2) fail(MDPD_E_ARG, "Invalid Masses line in data file");
+                    const int ty = (int)parse_int(f[0]);
+                    if (ty < 1 || ty > nt) fail(MDPD_E_ARG, "Invalid type for mass set");
+                    mass[ty] = parse_num(f[1]);
+                    if (mass[ty] <= 0) fail(MDPD_E_ARG, "Invalid mass value");
+                    mass_set[ty] = true;
+                } else if (sec == "Atoms") {
+                    if (f.size() != 5 && f.size() != 8) fail(MDPD_E_ARG, "Incorrect atom format in data file");
+                    const int tg = (int)parse_int(f[0]), ty = (int)parse_int(f[1]);
+                    if (tg <= 0) fail(MDPD_E_ARG, "Invalid atom ID in Atoms section of data file");
+                    if (ty <= 0 || ty > nt) fail(MDPD_E_ARG, "Invalid atom type in Atoms section of data file");
+                    where[tg] = h_tag.size();
+                    h_tag.push_back(tg);
+                    h_type.push_back(ty);
+                    h_mask.push_back(1);
+                    for (int d = 0; d < 3; ++d) h_x.push_back(parse_num(f[2 + d]));
+                    for (int d = 0; d < 3; ++d) h_image.push_back(f.size() == 8 ? (int)parse_int(f[5 + d]) : 0);
+                    for (int d = 0; d < 3; ++d) h_v.push_back(0.0);
+                } else {
+                    if (f.size() != 4) fail(MDPD_E_ARG, "Incorrect velocity format in data file");
+                    auto it = where.find((int)parse_int(f[0]));
+                    if (it == where.end()) fail(MDPD_E_ARG, "Invalid atom ID in Velocities section of data file");
+                    for (int d = 0; d < 3; ++d) h_v[3 * it->second + d] = parse_num(f[1 + d]);
+                }
+                ++rows;
+            }
+            if (rows < want) fail(MDPD_E_ARG, "Unexpected end of data file in section %s", sec.c_str());
+            while (std::getline(in, line)) {
+                const std::string t = strip(line);
+                if (t.empty()) continue;
+                std::istringstream is(t);
+                is >> section;
+                break;
+            }
+        }
+        if ((long long)(h_tag.size() - first) != natoms) fail(MDPD_E_ARG, "Did not assign all atoms correctly");
+        dirty = true;
+    }
+
+    // velocity G set vx vy vz (NULL keeps a component); src/velocity.cpp
+    void cmd_velocity(const std::vector<std::string> &w)
+    {
+        if (w.size() < 3) fail(MDPD_E_ARG, "Illegal velocity command");
+        const int bit = need_group(w[1]);
+        if (w[2] != "set") fail(MDPD_E_UNSUPPORTED, "velocity style %s is not supported", w[2].c_str());
+        if (w.size() < 6) fail(MDPD_E_ARG, "Illegal velocity command");
+        pull_state();
+        for (size_t i = 0; i < h_tag.size(); ++i) {
+            if (!(h_mask[i] & bit)) continue;
+            for (int d = 0; d < 3; ++d)
+                if (w[3 + d] != "NULL") h_v[3 * i + d] = parse_num(w[3 + d]);
+        }
+        dirty = true;
     }
 
     void cmd_group(const std::vector<std::string> &w)
@@ -354,6 +484,12 @@ struct mdpd_system {
                 }
             }
         } else if (w[2] == "empty") {
+        } else if (w[2] == "union") {
+            // group ID union G1 G2 ... (src/group.cpp)
+            int u = 0;
+            for (size_t k = 3; k < w.size(); ++k) u |= need_group(w[k]);
+            for (size_t i = 0; i < n; ++i)
+                if (h_mask[i] & u) h_mask[i] |= bit;
         } else {
             fail(MDPD_E_UNSUPPORTED, "group style %s is not supported", w[2].c_str());
         }
@@ -367,11 +503,20 @@ struct mdpd_system {
             // MesoPairMDPD::settings, pair_mdpd_meso.cu:299-328
             if (w.size() < 5)
                 fail(MDPD_E_ARG, "Illegal pair_style command:\n temperature cut_global seed fluid_group wall_group, or temperature cut_global seed ");
-            if (w.size() > 5) fail(MDPD_E_UNSUPPORTED, "the arbitrary-wall form of pair_style mdpd/meso is not built yet");
+            if (w.size() != 5 && w.size() != 7)
+                fail(MDPD_E_ARG, "Illegal pair_style command:\n temperature cut_global seed fluid_group wall_group, or temperature cut_global seed ");
             pair_style = STYLE_MDPD;
             temperature = parse_num(w[2]);
             cut_global = parse_num(w[3]);
             seed = (int)parse_int(w[4]);
+            pair_mobile_bit = pair_wall_bit = 0;
+            if (w.size() == 7) {
+                // arbitrary-wall form: fluid and wall groups (pair_mdpd_meso.cu:313-321)
+                pair_mobile_bit = group_bit(w[5]);
+                pair_wall_bit = group_bit(w[6]);
+                if (!pair_mobile_bit) fail(MDPD_E_ARG, "<MESO> Undefined fluid group id in pair_style mdpd/meso");
+                if (!pair_wall_bit) fail(MDPD_E_ARG, "<MESO> Undefined wall group id in pair_style mdpd/meso");
+            }
         } else if (w[1] == "dpd/meso") {
             // MesoPairDPD::settings, pair_dpd_meso.cu:272-287
             if (w.size() != 4) fail(MDPD_E_ARG, "Illegal pair_style command");
@@ -443,11 +588,26 @@ struct mdpd_system {
                     if (m == "lucy") f.lucy = 1;
                     else if (m == "quadratic") f.lucy = 0;
                     else fail(MDPD_E_ARG, "fix aboundary/rho/meso weight must be quadratic or lucy");
-                } else if (w[k] == "wall" || w[k] == "mobile" || w[k] == "cut_phi" || w[k] == "phi_c" || w[k] == "rho_wall") {
-                    fail(MDPD_E_UNSUPPORTED, "the arbitrary-boundary branch of fix aboundary/rho/meso is not built yet");
+                } else if (w[k] == "cut_phi" && k + 1 < w.size()) {
+                    f.cut_phi = parse_num(w[++k]);
+                } else if (w[k] == "phi_c" && k + 1 < w.size()) {
+                    f.phi_c = parse_num(w[++k]);
+                } else if (w[k] == "rho_wall" && k + 1 < w.size()) {
+                    f.rho_wall = parse_num(w[++k]);
+                } else if (w[k] == "wall" && k + 1 < w.size()) {
+                    f.wall_bit = group_bit(w[++k]);
+                    if (!f.wall_bit) fail(MDPD_E_ARG, "<MESO> Undefined wall group id in fix aboundary/rho/meso");
+                    f.arb = 1;
+                } else if (w[k] == "mobile" && k + 1 < w.size()) {
+                    f.mobile_bit = group_bit(w[++k]);
+                    if (!f.mobile_bit) fail(MDPD_E_ARG, "<MESO> Undefined fluid group id in fix aboundary/rho/meso");
+                    f.arb = 1;
                 }
             }
-            if (f.rd == 0) fail(MDPD_E_ARG, "<MESO> Fix aboundary/rho/meso parameters are not set properly.");
+            // fix_aboundary_rho_meso.cu:96-101
+            if (!f.arb && f.rd == 0) fail(MDPD_E_ARG, "<MESO> Fix aboundary/rho/meso parameters are not set properly.");
+            if (f.arb && (f.rd == 0 || f.cut_phi == 0 || f.phi_c == 0 || f.rho_wall == 0))
+                fail(MDPD_E_ARG, "<MESO> Fix aboundary/rho/meso parameters are not set properly.");
         } else if (f.style == "addforce/meso") {
             // MesoFixAddForce ctor, fix_addforce_meso.cu:38-47
             if (w.size() < 7) fail(MDPD_E_ARG, "Illegal fix addforce/meso command");
@@ -519,6 +679,7 @@ struct mdpd_system {
             P.tag[b] = tag[b].p; P.mask[b] = mask[b].p; P.origin[b] = origin[b].p; P.image[b] = image[b].p;
         }
         P.xhold = xhold.p; P.rho = rho.p; P.frc = frc.p; P.vir = vir.p; P.vend = vend.p;
+        P.phi[0] = phi[0].p; P.phi[1] = phi[1].p;
         P.pcell = pcell.p; P.pslot = pslot.p; P.psub = psub.p; P.tmp = tmp.p; P.key = key.p;
         P.cell_count = cell_count.p; P.cell_start = cell_start.p; P.cell_xyz = cell_xyz.p;
         P.cell_rank = cell_rank.p; P.cell_sys = cell_sys.p;
@@ -526,7 +687,7 @@ struct mdpd_system {
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
         P.cl = cl.p; P.wl = wl.p; P.work_total = work_total.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
-        P.cap = cap; P.ncellslots = nslots; P.nsys = 1; P.maxnbr = maxnbr;
+        P.cap = cap; P.nact = cap; P.ncellslots = nslots; P.nsys = 1; P.maxnbr = maxnbr;
         return P;
     }
 
@@ -605,6 +766,22 @@ struct mdpd_system {
             // fix_aboundary_rho_meso.cu:88-90
             d.rho_factor = fr->lucy ? (float)(105.0 / (16.0 * M_PI * r * r * r * r * r * r * r))
                                     : (float)(15.0 / (2.0 * M_PI * r * r * r * r * r));
+        }
+        if (pair_style == STYLE_MDPD && pair_mobile_bit && pair_wall_bit) {
+            d.pair_arb = 1;
+            d.pair_mobile_bit = pair_mobile_bit;
+            d.pair_wall_bit = pair_wall_bit;
+        }
+        if (fr && fr->arb) {
+            // fix_aboundary_rho_meso.cu:92-94
+            const double c = fr->cut_phi, c7 = c * c * c * c * c * c * c;
+            d.fix_arb = 1;
+            d.fix_mobile_bit = fr->mobile_bit;
+            d.fix_wall_bit = fr->wall_bit;
+            d.cut_phi = (float)c;
+            d.phi_c = (float)fr->phi_c;
+            d.phi_factor = (float)(105.0 / (16.0 * M_PI * fr->rho_wall * c7));
+            d.dw_factor = (float)(-315.0 / (4.0 * M_PI * fr->rho_wall * c7));
         }
         d.stale = 1;
         d.active = 1;
@@ -692,7 +869,7 @@ struct mdpd_system {
             tag[b].alloc(cap); mask[b].alloc(cap); origin[b].alloc(cap); image[b].alloc(cap);
         }
         xhold.alloc(cap); frc.alloc(cap); vend.alloc(cap);
-        rho.alloc(cap); vir.alloc((size_t)6 * cap);
+        rho.alloc(cap); vir.alloc((size_t)6 * cap); phi[0].alloc(cap); phi[1].alloc(cap);
         pcell.alloc(cap); pslot.alloc(cap); psub.alloc(cap); tmp.alloc(cap); key.alloc(2 * (size_t)cap);
         ncount.alloc(cap); ccount.alloc((size_t)4 * cap);
         nlist.alloc((size_t)cap * maxnbr);
@@ -728,6 +905,8 @@ struct mdpd_system {
         hip_check(hipMemsetAsync(work_total.p, 0, sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(ncount.p, 0, cap * sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(rho.p, 0, cap * sizeof(float), stream), "memset");
+        hip_check(hipMemsetAsync(phi[0].p, 0, cap * sizeof(float), stream), "memset");
+        hip_check(hipMemsetAsync(phi[1].p, 0, cap * sizeof(float), stream), "memset");
         hip_check(hipMemsetAsync(frc.p, 0, cap * sizeof(float4), stream), "memset");
         hip_check(hipMemsetAsync(vend.p, 0, cap * sizeof(float4), stream), "memset");
         hip_check(hipMemsetAsync(vir.p, 0, 6 * cap * sizeof(float), stream), "memset");
@@ -814,6 +993,8 @@ struct mdpd_system {
         const bool with_rho = pair_style == STYLE_MDPD && hsys.rho_bit;
         timed(PH_DENSITY, [&]() { hip_check(launch_first(P, with_rho, split, stream), "first pass"); });
         timed(PH_FORCE, [&]() { hip_check(launch_force(P, pair_style, mode, ntypes == 1, split, stream), "force"); });
+        if ((mode == MODE_FUSED || mode == MODE_FUSED_OUT) && hsys.fix_arb)
+            hip_check(launch_aboundary(P, 0, stream), "aboundary");          // post_integrate
         if (mode != MODE_SETUP) hip_check(launch_commit_step(P, mode != MODE_FINAL, split, stream), "commit");
     }
 
@@ -833,6 +1014,7 @@ struct mdpd_system {
         if (nsteps == 0) return;
         PoolDev P = view();
         timed(PH_INITIAL, [&]() { hip_check(launch_initial(P, stream), "initial"); });
+        if (hsys.fix_arb) hip_check(launch_aboundary(P, 0, stream), "aboundary");   // post_integrate
         hip_check(launch_commit_step(P, 1, split, stream), "commit");
         long long ago = 0;
         for (long long k = 1; k <= nsteps; ++k) {
@@ -910,6 +1092,11 @@ struct mdpd_system {
         } else if (name == "rho") {
             want(1, 1);
             std::vector<float> h = download(rho.p, cap);
+            hip_check(hipStreamSynchronize(stream), "sync");
+            for (int k = 0; k < n; ++k) put_d(k, 0, h[k]);
+        } else if (name == "phi") {
+            want(1, 1);
+            std::vector<float> h = download(phi[sela].p, cap);
             hip_check(hipStreamSynchronize(stream), "sync");
             for (int k = 0; k < n; ++k) put_d(k, 0, h[k]);
         } else if (name == "virial") {
