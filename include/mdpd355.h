@@ -94,7 +94,8 @@ long long mdpd_get_nbuild(const mdpd_system *sys);
 
 /* lammps_gather_atoms (src/library.h:41): copy a per-atom quantity in the
  * order atoms were created.  type 0 = int, 1 = double; count values per atom.
- * Names: "x" (1,3) "v" (1,3) "f" (1,3) "rho" (1,1) "e_pair" (1,1)
+ * Names: "x" (1,3) "v" (1,3) "f" (1,3) "rho" (1,1) "phi" (1,1: wall
+ * fraction of the arbitrary-boundary branch) "e_pair" (1,1)
  * "virial" (1,6: xx yy zz xy xz yz) "tag" (0,1) "type" (0,1) "mask" (0,1)
  * "image" (0,3).  f, e_pair and virial are those of the last force pass of
  * setup or of the last step of a run. */

@@ -233,3 +233,18 @@ Velocities
     with pytest.raises(engine.MdpdError) as e:
         t.command(f"read_data {tmp_path / 'missing.data'}")
     assert e.value.code == 1
+
+
+def test_every_gather_name_reaches_the_engine(engine):
+    # each per-atom name of the binding is known to the engine: before setup
+    # the engine answers with its own state error, not an unknown-name error
+    s = engine.System()
+    _deck(s)
+    s.create_atoms(np.ones(3, np.int32), np.zeros((3, 3)))
+    for name in engine._FIELDS:
+        if name in ("x", "v", "tag", "type", "mask"):
+            s.gather(name)
+            continue
+        with pytest.raises(engine.MdpdError) as e:
+            s.gather(name)
+        assert e.value.code == 2 and "only available after setup" in str(e.value)

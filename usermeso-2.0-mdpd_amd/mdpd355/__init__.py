@@ -122,7 +122,8 @@ def _ptr(a):
 
 _FIELDS = {  # name -> (type, count, dtype)
     "x": (1, 3, np.float64), "v": (1, 3, np.float64), "f": (1, 3, np.float64),
-    "rho": (1, 1, np.float64), "e_pair": (1, 1, np.float64), "virial": (1, 6, np.float64),
+    "rho": (1, 1, np.float64), "phi": (1, 1, np.float64), "e_pair": (1, 1, np.float64),
+    "virial": (1, 6, np.float64),
     "tag": (0, 1, np.int32), "type": (0, 1, np.int32), "mask": (0, 1, np.int32), "image": (0, 3, np.int32),
 }
 
