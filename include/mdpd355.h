@@ -51,8 +51,13 @@ const char *mdpd_version(void);
  * dimension 3, boundary p p p, atom_style mdpd/atomic/meso <rho_est> |
  * dpd/atomic/meso, mass, group ID type|id ..., pair_style mdpd/meso T rc seed,
  * pair_style dpd/meso rc seed, pair_coeff, fix ID G nve/meso,
- * fix ID G aboundary/rho/meso cut_rho rd [weight quadratic|lucy],
- * fix ID G addforce/meso fx fy fz, unfix, timestep, neighbor skin bin,
+ * fix ID G aboundary/rho/meso cut_rho rd [weight quadratic|lucy]
+ *   [cut_phi c phi_c p rho_wall w wall G mobile G],
+ * pair_style mdpd/meso T rc seed <mobile group> <wall group>,
+ * fix ID G addforce/meso fx fy fz, fix ID G rdf/fast/meso output F nbin N
+ * [every M] [other G], fix ID G density/meso output F along x|y|z nbin N
+ * every M window W, read_data F, write_data F, velocity G set vx vy vz,
+ * group ID union G..., unfix, timestep, neighbor skin bin,
  * neigh_modify every/delay/check, run_style mvv/meso, reset_timestep,
  * thermo, run N.  Comments (#) and blank lines are
  * ignored; echo/log/print/newton/comm_modify/atom_modify are accepted and
@@ -109,6 +114,17 @@ int mdpd_scatter_atoms(mdpd_system *sys, const char *name, int type, int count, 
  * out[3] kinetic energy.  compute_temp_meso.cu:49-101,
  * compute_pressure_meso.cu:115-155 */
 int mdpd_thermo(mdpd_system *sys, double out[4]);
+
+/* Current result of an observable fix, per bin: x = bin centre, value =
+ * g(r) for fix rdf/fast/meso (the reference's normalization with 3.1415,
+ * fix_rdf_fast_meso.cu:177-220; sampled after the force pass every `every`
+ * steps, fix_rdf_fast_meso.cu:140-175) or the mean count per bin of the
+ * current window for fix density/meso (fix_density_profile_meso.cpp:153-215).
+ * *nbin receives the bin count; with n = 0 only *nbin is written.  The fixes
+ * also write their output files as the reference does: rdf when the fix ends
+ * (unfix, redefinition, mdpd_close), density at the end of each window as
+ * <output>.<step>. */
+int mdpd_fix_result(mdpd_system *sys, const char *fix_id, int n, double *x, double *value, int *nbin);
 
 /* ---- tuning ----------------------------------------------------------------
  * Launch-shape knobs of the per-step passes.  No counterpart in the reference,

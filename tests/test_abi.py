@@ -233,6 +233,16 @@ Velocities
     with pytest.raises(engine.MdpdError) as e:
         t.command(f"read_data {tmp_path / 'missing.data'}")
     assert e.value.code == 1
+    # write_data -> read_data round trip: wrapped positions with image flags
+    out = tmp_path / "out.data"
+    s.command(f"write_data {out}")
+    r = engine.System()
+    r.command("atom_style mdpd/atomic/meso 3")
+    r.command(f"read_data {out}")
+    for name in ("tag", "type", "x", "v"):
+        np.testing.assert_array_equal(r.gather(name), s.gather(name))
+    line3 = next(l for l in out.read_text().splitlines() if l.startswith("3 1 "))
+    assert line3.endswith(" 0 0 1")                          # its image flags survive
 
 
 def test_every_gather_name_reaches_the_engine(engine):

@@ -18,7 +18,7 @@ OUT_DIR = os.path.join(HERE, "mdpd355")
 LIB = os.path.join(OUT_DIR, "libmdpd355.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["kernels_build.hip", "kernels_step.hip", "engine.cpp", "pool.cpp"]
+SOURCES = ["kernels_build.hip", "kernels_step.hip", "kernels_obs.hip", "engine.cpp", "pool.cpp"]
 HEADERS = ["common.hpp", "kernels.hpp", "system.hpp"]
 
 ARCH = "gfx950"

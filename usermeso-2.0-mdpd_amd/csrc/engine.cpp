@@ -188,6 +188,26 @@ int mdpd_thermo(mdpd_system *sys, double out[4])
     });
 }
 
+int mdpd_fix_result(mdpd_system *sys, const char *fix_id, int n, double *x, double *value, int *nbin)
+{
+    MDPD_GUARD(sys, {
+        if (!fix_id || !nbin) fail(MDPD_E_ARG, "null argument");
+        const FixDesc *f = nullptr;
+        for (auto &g : sys->fixes)
+            if (g.id == fix_id) f = &g;
+        if (!f) fail(MDPD_E_ARG, "could not find fix ID '%s'", fix_id);
+        if (!mdpd_system::is_obs(*f)) fail(MDPD_E_ARG, "fix %s has no result vector", fix_id);
+        *nbin = f->nbin;
+        if (n > 0) {
+            if (n < f->nbin || !x || !value) fail(MDPD_E_ARG, "result needs %d bins", f->nbin);
+            std::vector<double> xs, vs;
+            sys->obs_result(*f, xs, vs);
+            std::copy(xs.begin(), xs.end(), x);
+            std::copy(vs.begin(), vs.end(), value);
+        }
+    });
+}
+
 int mdpd_tune(mdpd_system *sys, const char *knob, int value)
 {
     MDPD_GUARD(sys, {

@@ -39,6 +39,10 @@ hipError_t launch_aboundary(const PoolDev &P, int pooled, hipStream_t st);
 // pooled engine, finished system s: current data into buffer 0 (sel = sela = 0)
 hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, hipStream_t st);
 
+// observables (accumulate into device histograms)
+hipError_t launch_rdf(const PoolDev &P, unsigned *hist, int nbin, float rc, int gi, int gj, hipStream_t st);
+hipError_t launch_profile(const PoolDev &P, unsigned *count, int nbin, int axis, int g, hipStream_t st);
+
 // stateless helpers for parity tests
 hipError_t launch_pair_gaussian(const uint32_t *a, const uint32_t *b, float *out, int n, hipStream_t st);
 hipError_t launch_premix(const uint32_t *a, const uint32_t *b, uint32_t *out, int n, int rounds, hipStream_t st);

@@ -1,0 +1,106 @@
+// Device observables of the mdpd355 engine: pair-distance histogram
+// (fix rdf/fast/meso) and binned number density along an axis
+// (fix density/meso).  Both count into LDS per workgroup and add the
+// workgroup's bins to the global histogram once (one atomic per bin per
+// workgroup).
+//
+// Replaces:
+//   MesoFixRDFFast::post_force / gpu_calc_rdf        fix_rdf_fast_meso.cu:102-175
+//   MesoFixDensityProf::compute / gpu_density_profile fix_density_profile_meso.cpp:118-190
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace mdpd {
+
+constexpr int OBS_BLK = 256;
+constexpr int OBS_MAXBIN = 4096;      // LDS bins per workgroup
+
+// pairs (i in group gi, j in group gj) of the neighbor rows with r < rc, at
+// the positions of the current buffer; bin = floor(r * nbin / rc).  Every
+// pair appears from both sides, as with the reference's full list.
+__global__ void __launch_bounds__(OBS_BLK) k_rdf(PoolDev P, unsigned *hist, int nbin, float rc, int gi, int gj)
+{
+    __shared__ unsigned h[OBS_MAXBIN];
+    for (int b = threadIdx.x; b < nbin; b += OBS_BLK) h[b] = 0u;
+    __syncthreads();
+    const int i = blockIdx.x * OBS_BLK + threadIdx.x;
+    if (i < P.nact) {
+        const int s = P.wave_sys[i >> 6];
+        const SysDev *d = s >= 0 ? P.sys + s : nullptr;
+        if (d && d->active && i < d->part_base + d->n && (P.mask[d->sela][i] & gi)) {
+            const float4 *pos = P.pos[d->sel];
+            const int *mask = P.mask[d->sela];
+            const float4 pi = pos[i];
+            const int maxnbr = P.maxnbr;
+            const int cnt = min(P.ncount[i], maxnbr);
+            const int *row = P.nlist + (size_t)(i >> 6) * maxnbr * WAVE + (i & 63);
+            const float rc2 = rc * rc, binv = (float)nbin / rc;
+            for (int q = 0; q < cnt; ++q) {
+                const uint32_t e = (uint32_t)row[(size_t)q * WAVE];
+                const uint32_t j = e & IDX_MASK;
+                if (!(mask[j] & gj)) continue;
+                const float4 pj = pos[j];
+                float xj = pj.x, yj = pj.y, zj = pj.z;
+                const uint32_t code = e >> IMG_SHIFT;
+                if (code != IMG_ZERO) {
+                    int sx, sy, sz;
+                    image_decode(code, sx, sy, sz);
+                    xj += (float)sx * d->L[0];
+                    yj += (float)sy * d->L[1];
+                    zj += (float)sz * d->L[2];
+                }
+                const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                const float rsq = dx * dx + dy * dy + dz * dz;
+                if (rsq < rc2) {
+                    const int b = min((int)floorf(rsq * __builtin_amdgcn_rsqf(rsq) * binv), nbin - 1);
+                    atomicAdd(&h[b], 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbin; b += OBS_BLK)
+        if (h[b]) atomicAdd(hist + b, h[b]);
+}
+
+// particles of group g binned by coordinate `axis` of the current buffer:
+// bin = floor((x - lo) * nbin / L); out-of-range coordinates (a particle
+// past the box between re-sorts) are not counted
+__global__ void __launch_bounds__(OBS_BLK) k_profile(PoolDev P, unsigned *count, int nbin, int axis, int g)
+{
+    __shared__ unsigned h[OBS_MAXBIN];
+    for (int b = threadIdx.x; b < nbin; b += OBS_BLK) h[b] = 0u;
+    __syncthreads();
+    const int i = blockIdx.x * OBS_BLK + threadIdx.x;
+    if (i < P.nact) {
+        const int s = P.wave_sys[i >> 6];
+        const SysDev *d = s >= 0 ? P.sys + s : nullptr;
+        if (d && d->active && i < d->part_base + d->n && (P.mask[d->sela][i] & g)) {
+            const float4 p = P.pos[d->sel][i];
+            const float c = axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
+            const int b = (int)floorf((c - d->lo[axis]) * ((float)nbin / d->L[axis]));
+            if (b >= 0 && b < nbin) atomicAdd(&h[b], 1u);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbin; b += OBS_BLK)
+        if (h[b]) atomicAdd(count + b, h[b]);
+}
+
+hipError_t launch_rdf(const PoolDev &P, unsigned *hist, int nbin, float rc, int gi, int gj, hipStream_t st)
+{
+    if (nbin < 1 || nbin > OBS_MAXBIN) return hipErrorInvalidValue;
+    if (P.nact == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rdf, dim3((P.nact + OBS_BLK - 1) / OBS_BLK), dim3(OBS_BLK), 0, st, P, hist, nbin, rc, gi, gj);
+    return hipGetLastError();
+}
+
+hipError_t launch_profile(const PoolDev &P, unsigned *count, int nbin, int axis, int g, hipStream_t st)
+{
+    if (nbin < 1 || nbin > OBS_MAXBIN || axis < 0 || axis > 2) return hipErrorInvalidValue;
+    if (P.nact == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_profile, dim3((P.nact + OBS_BLK - 1) / OBS_BLK), dim3(OBS_BLK), 0, st, P, count, nbin, axis, g);
+    return hipGetLastError();
+}
+
+}  // namespace mdpd

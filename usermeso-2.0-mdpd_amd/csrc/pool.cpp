@@ -303,6 +303,9 @@ struct mdpd_pool {
         if (!sys) fail(MDPD_E_ARG, "null system");
         if (nsteps < 1) fail(MDPD_E_ARG, "a job needs at least one step");
         if (!sys->run_style_ok) fail(MDPD_E_STATE, "run_style mvv/meso must be set for the USER-MESO styles");
+        for (auto &f : sys->fixes)
+            if (mdpd_system::is_obs(f))
+                fail(MDPD_E_UNSUPPORTED, "fix %s (%s) is not available for pool jobs", f.id.c_str(), f.style.c_str());
         if (sys->setup_done) sys->pull_state();
         auto j = std::make_unique<Job>();
         sys->describe(j->ds);

@@ -122,6 +122,17 @@ struct FixDesc {
     double cut_phi = 0, phi_c = 0, rho_wall = 0;
     int wall_bit = 0, mobile_bit = 0, arb = 0;
     double f[3] = {0, 0, 0};   // addforce/meso
+    // rdf/fast/meso (fix_rdf_fast_meso.cu:37-75) and density/meso
+    // (fix_density_profile_meso.cpp:30-83)
+    std::string output;
+    int nbin = 0, every = 1, window = 0, along = -1, other_bit = 0;
+};
+
+// device histogram and sample counts of an observable fix
+struct ObsState {
+    DevBuf<unsigned> hist;
+    long long samples = 0;       // rdf: sampled steps; density: frames of the current window
+    long long last_dump = -1;
 };
 
 // Thermo scalars of one system (compute_temp_meso.cu:49-101: dof = 3N - 3;
@@ -176,6 +187,7 @@ struct mdpd_system {
     int every = 1, delay = 10, check = 1;
     bool run_style_ok = false;
     std::vector<FixDesc> fixes;
+    std::map<std::string, ObsState> obs;
     std::vector<std::pair<std::string, int>> groups{{"all", 1}};
     long long ntimestep = 0;
     int thermo_every = 0;
@@ -209,6 +221,9 @@ struct mdpd_system {
 
     ~mdpd_system()
     {
+        for (auto &f : fixes) {
+            try { finish_obs(f); } catch (...) {}
+        }
         for (auto &e : ev_open) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
         for (auto e : ev_free) (void)hipEventDestroy(e);
         if (pinned) (void)hipHostFree(pinned);
@@ -296,8 +311,10 @@ struct mdpd_system {
             need(2);
             auto it = std::find_if(fixes.begin(), fixes.end(), [&](const FixDesc &f) { return f.id == w[1]; });
             if (it == fixes.end()) fail(MDPD_E_ARG, "could not find fix ID '%s' to delete", w[1].c_str());
+            const bool o = is_obs(*it);
+            finish_obs(*it);
             fixes.erase(it);
-            dirty = true;
+            if (!o) dirty = true;
         } else if (c == "timestep") {
             need(2);
             dt = parse_num(w[1]);
@@ -329,6 +346,9 @@ struct mdpd_system {
         } else if (c == "read_data") {
             need(2);
             read_data(w[1]);
+        } else if (c == "write_data") {
+            need(2);
+            write_data(w[1]);
         } else if (c == "velocity") {
             cmd_velocity(w);
         } else if (c == "thermo") {
@@ -444,6 +464,42 @@ struct mdpd_system {
         }
         if ((long long)(h_tag.size() - first) != natoms) fail(MDPD_E_ARG, "Did not assign all atoms correctly");
         dirty = true;
+    }
+
+    // write_data: header, Masses, Atoms (id type x y z ix iy iz, positions
+    // wrapped into the box with the image flags adjusted) and Velocities, in
+    // creation order; src/write_data.cpp
+    void write_data(const std::string &path)
+    {
+        if (!box_set) fail(MDPD_E_STATE, "Write_data command before simulation box is defined");
+        pull_state();
+        FILE *f = fopen(path.c_str(), "w");
+        if (!f) fail(MDPD_E_ARG, "Cannot open data file %s", path.c_str());
+        const size_t n = h_tag.size();
+        fprintf(f, "LAMMPS data file via write_data (mdpd355), timestep = %lld\n\n", ntimestep);
+        fprintf(f, "%zu atoms\n%d atom types\n\n", n, ntypes);
+        const char *ax[3] = {"x", "y", "z"};
+        for (int d = 0; d < 3; ++d) fprintf(f, "%.16e %.16e %slo %shi\n", lo[d], hi[d], ax[d], ax[d]);
+        fprintf(f, "\nMasses\n\n");
+        for (int t = 1; t <= ntypes; ++t) fprintf(f, "%d %.16g\n", t, mass_set[t] ? mass[t] : 0.0);
+        fprintf(f, "\nAtoms # %s\n\n", atom_style == STYLE_DPD ? "dpd/atomic/meso" : "mdpd/atomic/meso");
+        for (size_t i = 0; i < n; ++i) {
+            double x[3];
+            int im[3];
+            for (int d = 0; d < 3; ++d) {
+                const double L = hi[d] - lo[d];
+                const double k = std::floor((h_x[3 * i + d] - lo[d]) / L);
+                x[d] = h_x[3 * i + d] - k * L;
+                if (x[d] >= hi[d]) x[d] = lo[d];
+                im[d] = h_image[3 * i + d] + (int)k;
+            }
+            fprintf(f, "%d %d %.16e %.16e %.16e %d %d %d\n", h_tag[i], h_type[i], x[0], x[1], x[2], im[0], im[1], im[2]);
+        }
+        fprintf(f, "\nVelocities\n\n");
+        for (size_t i = 0; i < n; ++i)
+            fprintf(f, "%d %.16e %.16e %.16e\n", h_tag[i], h_v[3 * i], h_v[3 * i + 1], h_v[3 * i + 2]);
+        const bool ok = fclose(f) == 0;
+        if (!ok) fail(MDPD_E_ARG, "error writing data file %s", path.c_str());
     }
 
     // velocity G set vx vy vz (NULL keeps a component); src/velocity.cpp
@@ -608,6 +664,34 @@ struct mdpd_system {
             if (!f.arb && f.rd == 0) fail(MDPD_E_ARG, "<MESO> Fix aboundary/rho/meso parameters are not set properly.");
             if (f.arb && (f.rd == 0 || f.cut_phi == 0 || f.phi_c == 0 || f.rho_wall == 0))
                 fail(MDPD_E_ARG, "<MESO> Fix aboundary/rho/meso parameters are not set properly.");
+        } else if (f.style == "rdf/fast/meso") {
+            // MesoFixRDFFast ctor: output FILE nbin N [every M] [other G]
+            for (size_t k = 4; k < w.size(); ++k) {
+                if (w[k] == "output" && k + 1 < w.size()) f.output = w[++k];
+                else if (w[k] == "nbin" && k + 1 < w.size()) f.nbin = (int)parse_int(w[++k]);
+                else if (w[k] == "every" && k + 1 < w.size()) f.every = (int)parse_int(w[++k]);
+                else if (w[k] == "other" && k + 1 < w.size()) {
+                    f.other_bit = group_bit(w[++k]);
+                    if (!f.other_bit) fail(MDPD_E_ARG, "<MESO> Undefined other group id in fix rdf/meso");
+                }
+            }
+            if (f.output.empty() || f.nbin <= 0) fail(MDPD_E_ARG, "Incomplete compute rdf command: insufficient arguments");
+            if (f.nbin > 4096 || f.every < 1) fail(MDPD_E_ARG, "fix rdf/fast/meso: nbin must be 1..4096 and every >= 1");
+            if (!f.other_bit) f.other_bit = f.groupbit;
+        } else if (f.style == "density/meso") {
+            // MesoFixDensityProf ctor: output FILE along x|y|z nbin N every M window W
+            for (size_t k = 4; k < w.size(); ++k) {
+                if (w[k] == "output" && k + 1 < w.size()) f.output = w[++k];
+                else if (w[k] == "along" && k + 1 < w.size()) {
+                    const char a = w[++k][0];
+                    f.along = a == 'x' ? 0 : (a == 'y' ? 1 : (a == 'z' ? 2 : -1));
+                } else if (w[k] == "nbin" && k + 1 < w.size()) f.nbin = (int)parse_int(w[++k]);
+                else if (w[k] == "every" && k + 1 < w.size()) f.every = (int)parse_int(w[++k]);
+                else if (w[k] == "window" && k + 1 < w.size()) f.window = (int)parse_int(w[++k]);
+            }
+            if (f.output.empty() || f.along < 0 || f.nbin <= 0 || f.every <= 0 || f.window <= 0)
+                fail(MDPD_E_ARG, "Incomplete fix density/meso command: insufficient arguments");
+            if (f.nbin > 4096) fail(MDPD_E_ARG, "fix density/meso: nbin must be 1..4096");
         } else if (f.style == "addforce/meso") {
             // MesoFixAddForce ctor, fix_addforce_meso.cu:38-47
             if (w.size() < 7) fail(MDPD_E_ARG, "Illegal fix addforce/meso command");
@@ -616,9 +700,147 @@ struct mdpd_system {
             fail(MDPD_E_UNSUPPORTED, "fix style %s is outside this engine", f.style.c_str());
         }
         auto it = std::find_if(fixes.begin(), fixes.end(), [&](const FixDesc &g) { return g.id == f.id; });
-        if (it != fixes.end()) *it = f;
-        else fixes.push_back(f);
+        if (it != fixes.end()) {
+            finish_obs(*it);
+            *it = f;
+        } else {
+            fixes.push_back(f);
+        }
+        if (is_obs(f)) {
+            obs.erase(f.id);
+            ObsState &o = obs[f.id];
+            (void)o;
+            return;                      // observables do not change the step: no new setup
+        }
         dirty = true;
+    }
+
+    // ------------------------------------------------------------------
+    // observables: rdf/fast/meso, density/meso
+    // ------------------------------------------------------------------
+    static bool is_obs(const FixDesc &f) { return f.style == "rdf/fast/meso" || f.style == "density/meso"; }
+
+    ObsState &obs_state(const FixDesc &f)
+    {
+        ObsState &o = obs[f.id];
+        if (!o.hist.p) {
+            ensure_stream();
+            hip_check(hipSetDevice(device), "hipSetDevice");
+            o.hist.alloc(f.nbin);
+            hip_check(hipMemsetAsync(o.hist.p, 0, sizeof(unsigned) * f.nbin, stream), "memset");
+        }
+        return o;
+    }
+
+    std::vector<unsigned> obs_counts(ObsState &o, int nbin)
+    {
+        std::vector<unsigned> h(nbin, 0u);
+        if (o.hist.p) {
+            hip_check(hipMemcpyAsync(h.data(), o.hist.p, sizeof(unsigned) * nbin, hipMemcpyDeviceToHost, stream), "download");
+            hip_check(hipStreamSynchronize(stream), "sync");
+        }
+        return h;
+    }
+
+    // current result of an observable fix: bin centres and values (rdf: g(r)
+    // with the reference's normalization, 4/3 * 3.1415 * shell volume,
+    // fix_rdf_fast_meso.cu:187-217; density: counts per bin averaged over
+    // the frames of the current window, fix_density_profile_meso.cpp:192-215)
+    void obs_result(const FixDesc &f, std::vector<double> &x, std::vector<double> &val)
+    {
+        ObsState &o = obs[f.id];
+        std::vector<unsigned> h = obs_counts(o, f.nbin);
+        x.assign(f.nbin, 0.0);
+        val.assign(f.nbin, 0.0);
+        if (f.style == "rdf/fast/meso") {
+            const double rc = max_cut(), bin = rc / f.nbin;
+            long long ni = 0, nj = 0;
+            for (size_t i = 0; i < h_mask.size(); ++i) {
+                ni += (h_mask[i] & f.groupbit) != 0;
+                nj += (h_mask[i] & f.other_bit) != 0;
+            }
+            const double V = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+            const double rho_j = nj / V;
+            for (int b = 0; b < f.nbin; ++b) {
+                x[b] = (b + 0.5) * bin;
+                if (!o.samples || !ni || !nj) continue;
+                const double freq = h[b] / double(ni) / double(o.samples);
+                const double shell = 4. / 3. * 3.1415 * (std::pow(bin * (b + 1), 3.) - std::pow(bin * b, 3.));
+                val[b] = freq / shell / rho_j;
+            }
+        } else {
+            const double bin = (hi[f.along] - lo[f.along]) / f.nbin;
+            for (int b = 0; b < f.nbin; ++b) {
+                x[b] = (b + 0.5) * bin;
+                val[b] = o.samples ? h[b] / double((float)o.samples) : 0.0;
+            }
+        }
+    }
+
+    void write_obs(const FixDesc &f, const std::string &path)
+    {
+        std::vector<double> x, val;
+        obs_result(f, x, val);
+        FILE *fp = fopen(path.c_str(), "w");
+        if (!fp) fail(MDPD_E_ARG, "Cannot open %s", path.c_str());
+        for (int b = 0; b < f.nbin; ++b) fprintf(fp, "%.15g\t%.15g\t\n", x[b], val[b]);
+        fclose(fp);
+    }
+
+    // per step t of a run: density/meso samples x(t) before the force pass
+    // (post_integrate), rdf/fast/meso after it (post_force)
+    void obs_pre(long long t)
+    {
+        for (auto &f : fixes) {
+            if (f.style != "density/meso") continue;
+            ObsState &o = obs_state(f);
+            const long long n = (t + f.every / 2) / f.every, m = n * f.every;
+            if (t - m >= -f.window && t - m < f.window) {
+                PoolDev P = view();
+                hip_check(launch_profile(P, o.hist.p, f.nbin, f.along, f.groupbit, stream), "density profile");
+                ++o.samples;
+            } else if (t - m == f.window) {
+                dump_profile(f, o, m, t);
+            }
+        }
+    }
+
+    void obs_post(long long t)
+    {
+        for (auto &f : fixes) {
+            if (f.style != "rdf/fast/meso" || t % f.every != 0) continue;
+            ObsState &o = obs_state(f);
+            PoolDev P = view();
+            hip_check(launch_rdf(P, o.hist.p, f.nbin, (float)max_cut(), f.groupbit, f.other_bit, stream), "rdf");
+            ++o.samples;
+        }
+    }
+
+    void dump_profile(const FixDesc &f, ObsState &o, long long m, long long t)
+    {
+        char fn[512];
+        snprintf(fn, sizeof(fn), "%s.%09lld", f.output.c_str(), m);
+        write_obs(f, fn);
+        o.last_dump = t;
+        o.samples = 0;
+        hip_check(hipMemsetAsync(o.hist.p, 0, sizeof(unsigned) * f.nbin, stream), "memset");
+    }
+
+    // end of an observable fix (unfix, redefinition, handle closed): the
+    // reference's destructors write the accumulated result
+    void finish_obs(const FixDesc &f)
+    {
+        if (!is_obs(f)) return;
+        auto it = obs.find(f.id);
+        if (it == obs.end() || !it->second.hist.p) return;
+        ObsState &o = it->second;
+        if (f.style == "rdf/fast/meso") {
+            write_obs(f, f.output);
+        } else {
+            const long long t = ntimestep, n = (t + f.every / 2) / f.every, m = n * f.every;
+            if (t - m >= -f.window && t - m < f.window && o.last_dump < t && o.samples) dump_profile(f, o, m, t);
+        }
+        obs.erase(it);
     }
 
     // ------------------------------------------------------------------
@@ -987,12 +1209,15 @@ struct mdpd_system {
         return pinned[0];
     }
 
-    void force_pass(int mode)
+    // t: timestep this pass evaluates (observables sample it), -1 at setup
+    void force_pass(int mode, long long t = -1)
     {
         PoolDev P = view();
         const bool with_rho = pair_style == STYLE_MDPD && hsys.rho_bit;
+        if (t >= 0 && !obs.empty()) obs_pre(t);
         timed(PH_DENSITY, [&]() { hip_check(launch_first(P, with_rho, split, stream), "first pass"); });
         timed(PH_FORCE, [&]() { hip_check(launch_force(P, pair_style, mode, ntypes == 1, split, stream), "force"); });
+        if (t >= 0 && !obs.empty()) obs_post(t);
         if ((mode == MODE_FUSED || mode == MODE_FUSED_OUT) && hsys.fix_arb)
             hip_check(launch_aboundary(P, 0, stream), "aboundary");          // post_integrate
         if (mode != MODE_SETUP) hip_check(launch_commit_step(P, mode != MODE_FINAL, split, stream), "commit");
@@ -1029,7 +1254,7 @@ struct mdpd_system {
                 rebuild();
                 ago = 0;
             }
-            force_pass(k < nsteps ? MODE_FUSED : MODE_FINAL);
+            force_pass(k < nsteps ? MODE_FUSED : MODE_FINAL, ntimestep + k);
         }
         hip_check(hipStreamSynchronize(stream), "run");
         collect_events();

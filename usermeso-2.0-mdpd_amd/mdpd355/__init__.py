@@ -84,6 +84,7 @@ def load():
         "mdpd_scatter_atoms": (I, [P, ctypes.c_char_p, I, I, P]),
         "mdpd_thermo": (I, [P, ctypes.POINTER(D)]),
         "mdpd_tune": (I, [P, ctypes.c_char_p, I]),
+        "mdpd_fix_result": (I, [P, ctypes.c_char_p, I, P, P, ctypes.POINTER(I)]),
         "mdpd_profile": (I, [P, I]),
         "mdpd_profile_get": (I, [P, ctypes.c_char_p, ctypes.POINTER(D), ctypes.POINTER(LL)]),
         "mdpd_debug_sorted": (I, [P, P, P, P]),
@@ -223,6 +224,15 @@ class System:
         out = (ctypes.c_double * 4)()
         self._check(self._lib.mdpd_thermo(self._h, out))
         return {"temp": out[0], "press": out[1], "pe": out[2], "ke": out[3]}
+
+    def fix_result(self, fix_id):
+        """(bin centres, values) of an observable fix (rdf/fast/meso, density/meso)."""
+        nb = ctypes.c_int()
+        self._check(self._lib.mdpd_fix_result(self._h, fix_id.encode(), 0, None, None, ctypes.byref(nb)))
+        x = np.empty(nb.value)
+        v = np.empty(nb.value)
+        self._check(self._lib.mdpd_fix_result(self._h, fix_id.encode(), nb.value, _ptr(x), _ptr(v), ctypes.byref(nb)))
+        return x, v
 
     # -- tuning / kernel timing --
     def tune(self, knob, value):
