@@ -258,3 +258,27 @@ def test_every_gather_name_reaches_the_engine(engine):
         with pytest.raises(engine.MdpdError) as e:
             s.gather(name)
         assert e.value.code == 2 and "only available after setup" in str(e.value)
+
+
+def test_equal_style_variables(engine):
+    # variable NAME equal FORMULA; ${NAME} and $x substitute its value
+    s = engine.System()
+    s.commands("""
+        variable    cut_att   equal 1.00
+        variable    skin      equal 0.3*${cut_att}
+        variable    x         equal -(2^3 - 1)/2 + ${skin}
+        units       lj
+        atom_style  mdpd/atomic/meso 3
+        neighbor    ${skin} bin
+    """)
+    s.create_box(1, (0, 0, 0), (5, 5, 5))
+    s.command("mass 1 ${cut_att}")
+    with pytest.raises(engine.MdpdError) as e:
+        s.command("neighbor ${undefined} bin")
+    assert e.value.code == 1 and "illegal variable" in str(e.value)
+    with pytest.raises(engine.MdpdError):
+        s.command("variable bad equal 1+(2")
+        s.command("mass 1 ${bad}")
+    with pytest.raises(engine.MdpdError) as e:
+        s.command("variable s string abc")
+    assert e.value.code == 6

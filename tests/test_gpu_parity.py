@@ -397,3 +397,35 @@ def test_observable_fixes_count_what_the_host_counts(engine, tmp_path):
     s.command("unfix g")
     out = np.loadtxt(tmp_path / "rdf.out")
     assert out.shape == (20, 2) and np.allclose(out[:, 1], g, rtol=1e-12)
+
+
+def _read_dump(path):
+    frames, lines, k = {}, open(path).read().split("\n"), 0
+    while k < len(lines) and lines[k].startswith("ITEM: TIMESTEP"):
+        t = int(lines[k + 1]); n = int(lines[k + 3]); cols = lines[k + 8].split()[2:]
+        rows = np.array([[float(c) for c in l.split()] for l in lines[k + 9:k + 9 + n]])
+        frames[t] = dict(zip(cols, rows.T))
+        k += 9 + n
+    return frames
+
+
+def test_dump_custom_frames_match_the_state_of_their_step(engine, tmp_path):
+    n, L = 1296, 6.0
+    x, v = uniform_box(n, L, 31)
+    f = tmp_path / "traj.dump"
+    s = mdpd_system(engine, x, v, L, extra=f"dump d all custom 5 {f} id type x y z vx vy vz fx fy fz")
+    s.run(10)
+    fr = _read_dump(f)
+    assert sorted(fr) == [0, 5, 10]
+    assert np.array_equal(fr[0]["id"], np.arange(1, n + 1))
+    # step 0: the setup state; step 10: the end of the run
+    np.testing.assert_array_equal(np.c_[fr[0]["x"], fr[0]["y"], fr[0]["z"]], x.astype(np.float32))
+    for key, name in (("x", "x"), ("vx", "v"), ("fx", "f")):
+        got = np.c_[fr[10][key], fr[10][key.replace("x", "y", 1) if key != "x" else "y"],
+                    fr[10][key.replace("x", "z", 1) if key != "x" else "z"]]
+        np.testing.assert_allclose(got, s.gather(name), rtol=1e-7, atol=1e-7)
+    # step 5 (written from inside the fused step loop) equals a run of 5
+    r = mdpd_system(engine, x, v, L)
+    r.run(5)
+    np.testing.assert_allclose(np.c_[fr[5]["x"], fr[5]["y"], fr[5]["z"]], r.gather("x"), rtol=1e-7, atol=1e-7)
+    np.testing.assert_allclose(np.c_[fr[5]["vx"], fr[5]["vy"], fr[5]["vz"]], r.gather("v"), rtol=1e-7, atol=1e-7)

@@ -122,16 +122,24 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
             ++nc;
         }
     };
+    // four entries (and their position gathers) in flight per lane, visited
+    // in row order
     int p = w;
-    for (; p + SPLIT < cnt; p += 2 * SPLIT) {
+    for (; p + 3 * SPLIT < cnt; p += 4 * SPLIT) {
         const uint32_t e0 = (uint32_t)row[(size_t)p * WAVE];
         const uint32_t e1 = (uint32_t)row[(size_t)(p + SPLIT) * WAVE];
+        const uint32_t e2 = (uint32_t)row[(size_t)(p + 2 * SPLIT) * WAVE];
+        const uint32_t e3 = (uint32_t)row[(size_t)(p + 3 * SPLIT) * WAVE];
         const float4 q0 = pos[e0 & IDX_MASK];
         const float4 q1 = pos[e1 & IDX_MASK];
+        const float4 q2 = pos[e2 & IDX_MASK];
+        const float4 q3 = pos[e3 & IDX_MASK];
         visit(e0, q0);
         visit(e1, q1);
+        visit(e2, q2);
+        visit(e3, q3);
     }
-    if (p < cnt) {
+    for (; p < cnt; p += SPLIT) {
         const uint32_t e0 = (uint32_t)row[(size_t)p * WAVE];
         visit(e0, pos[e0 & IDX_MASK]);
     }
@@ -326,6 +334,31 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
     const int *crow = P.clist + (size_t)tile * maxnbr * WAVE + lane;
     Acc<EV> a;
     int t = 0;
+    if (UNI) {
+        // single type: one coefficient set, four pairs' gathers in flight
+        for (; t + 4 <= cnt; t += 4) {
+            uint32_t e[4], j[4];
+            float4 q[4], vv[4];
+            float rj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                e[u] = (uint32_t)crow[(size_t)((t + u) * SPLIT + w) * WAVE];
+                j[u] = e[u] & IDX_MASK;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                q[u] = pos[j[u]];
+                vv[u] = vel[j[u]];
+                rj[u] = STYLE == STYLE_MDPD ? P.rho[j[u]] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                PairC c = cu;
+                wall_scale(c, j[u]);
+                pair_term<STYLE, EV>(a, pi, vi, rho_i, q[u], vv[u], rj[u], e[u] >> IMG_SHIFT, c, dtis, d);
+            }
+        }
+    }
     for (; t + 2 <= cnt; t += 2) {
         const uint32_t e0 = (uint32_t)crow[(size_t)(t * SPLIT + w) * WAVE];
         const uint32_t e1 = (uint32_t)crow[(size_t)((t + 1) * SPLIT + w) * WAVE];

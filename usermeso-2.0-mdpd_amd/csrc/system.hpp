@@ -26,6 +26,7 @@
 #include <map>
 #include <numeric>
 #include <fstream>
+#include <functional>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -128,6 +129,15 @@ struct FixDesc {
     int nbin = 0, every = 1, window = 0, along = -1, other_bit = 0;
 };
 
+// dump ID group custom N file columns... (src/dump_custom.cpp)
+struct DumpDesc {
+    std::string id, file;
+    int groupbit = 1;
+    long long every = 1;
+    std::vector<std::string> cols;
+    bool started = false;
+};
+
 // device histogram and sample counts of an observable fix
 struct ObsState {
     DevBuf<unsigned> hist;
@@ -188,6 +198,7 @@ struct mdpd_system {
     bool run_style_ok = false;
     std::vector<FixDesc> fixes;
     std::map<std::string, ObsState> obs;
+    std::vector<DumpDesc> dumps;
     std::vector<std::pair<std::string, int>> groups{{"all", 1}};
     long long ntimestep = 0;
     int thermo_every = 0;
@@ -253,9 +264,127 @@ struct mdpd_system {
     // ------------------------------------------------------------------
     // input-script surface
     // ------------------------------------------------------------------
+    // ------------------------------------------------------------------
+    // equal-style variables (src/variable.cpp): `variable NAME equal EXPR`
+    // stores the formula; ${NAME} / $x in later lines is replaced by its
+    // value (numbers, + - * / ^, parentheses, unary minus, other variables)
+    // ------------------------------------------------------------------
+    std::map<std::string, std::string> variables;
+
+    double eval_formula(const std::string &text, int depth = 0)
+    {
+        if (depth > 32) fail(MDPD_E_ARG, "variable formula nests too deeply");
+        const std::string e = substitute(text, depth + 1);
+        size_t k = 0;
+        auto skip = [&]() { while (k < e.size() && isspace((unsigned char)e[k])) ++k; };
+        std::function<double()> expr, term, power, unary, atom;
+        atom = [&]() -> double {
+            skip();
+            if (k < e.size() && e[k] == '(') {
+                ++k;
+                const double v = expr();
+                skip();
+                if (k >= e.size() || e[k] != ')') fail(MDPD_E_ARG, "unbalanced parentheses in '%s'", e.c_str());
+                ++k;
+                return v;
+            }
+            const char *b = e.c_str() + k;
+            char *end = nullptr;
+            const double v = strtod(b, &end);
+            if (end == b) fail(MDPD_E_ARG, "invalid variable formula '%s'", e.c_str());
+            k += (size_t)(end - b);
+            return v;
+        };
+        unary = [&]() -> double {
+            skip();
+            if (k < e.size() && (e[k] == '-' || e[k] == '+')) {
+                const bool neg = e[k] == '-';
+                ++k;
+                const double v = unary();
+                return neg ? -v : v;
+            }
+            return atom();
+        };
+        power = [&]() -> double {
+            const double b = unary();
+            skip();
+            if (k < e.size() && e[k] == '^') { ++k; return std::pow(b, power()); }
+            return b;
+        };
+        term = [&]() -> double {
+            double v = power();
+            for (;;) {
+                skip();
+                if (k < e.size() && (e[k] == '*' || e[k] == '/')) {
+                    const char op = e[k++];
+                    const double r = power();
+                    v = op == '*' ? v * r : v / r;
+                } else {
+                    return v;
+                }
+            }
+        };
+        expr = [&]() -> double {
+            double v = term();
+            for (;;) {
+                skip();
+                if (k < e.size() && (e[k] == '+' || e[k] == '-')) {
+                    const char op = e[k++];
+                    const double r = term();
+                    v = op == '+' ? v + r : v - r;
+                } else {
+                    return v;
+                }
+            }
+        };
+        const double v = expr();
+        skip();
+        if (k != e.size()) fail(MDPD_E_ARG, "invalid variable formula '%s'", e.c_str());
+        return v;
+    }
+
+    std::string substitute(const std::string &line, int depth = 0)
+    {
+        std::string out;
+        for (size_t k = 0; k < line.size(); ++k) {
+            if (line[k] != '$' || k + 1 >= line.size()) { out += line[k]; continue; }
+            std::string name;
+            if (line[k + 1] == '{') {
+                const size_t close = line.find('}', k + 2);
+                if (close == std::string::npos) fail(MDPD_E_ARG, "invalid variable name in '%s'", line.c_str());
+                name = line.substr(k + 2, close - k - 2);
+                k = close;
+            } else {
+                name = line.substr(k + 1, 1);
+                k += 1;
+            }
+            auto it = variables.find(name);
+            if (it == variables.end()) fail(MDPD_E_ARG, "Substitution for illegal variable %s", name.c_str());
+            char buf[64];
+            snprintf(buf, sizeof(buf), "%.15g", eval_formula(it->second, depth));
+            out += buf;
+        }
+        return out;
+    }
+
     void command(const std::string &raw)
     {
         std::string line = raw.substr(0, raw.find('#'));
+        {
+            // `variable NAME equal FORMULA` keeps its formula text unevaluated
+            std::istringstream vs(line);
+            std::string c0, nm, style;
+            vs >> c0 >> nm >> style;
+            if (c0 == "variable") {
+                if (style != "equal") fail(MDPD_E_UNSUPPORTED, "only equal-style variables are supported");
+                std::string rest;
+                std::getline(vs, rest);
+                if (rest.find_first_not_of(" \t\r") == std::string::npos) fail(MDPD_E_ARG, "Illegal variable command");
+                variables[nm] = rest;
+                return;
+            }
+        }
+        if (line.find('$') != std::string::npos) line = substitute(line);
         std::istringstream is(line);
         std::vector<std::string> w;
         for (std::string t; is >> t;) w.push_back(t);
@@ -265,7 +394,7 @@ struct mdpd_system {
             if (w.size() < n) fail(MDPD_E_ARG, "illegal %s command", c.c_str());
         };
         if (c == "echo" || c == "log" || c == "print" || c == "newton" || c == "comm_modify" ||
-            c == "atom_modify" || c == "processors") {
+            c == "atom_modify" || c == "processors" || c == "thermo_style" || c == "thermo_modify") {
             return;
         } else if (c == "units") {
             need(2);
@@ -351,6 +480,14 @@ struct mdpd_system {
             write_data(w[1]);
         } else if (c == "velocity") {
             cmd_velocity(w);
+        } else if (c == "dump") {
+            cmd_dump(w);
+        } else if (c == "undump") {
+            need(2);
+            auto it = std::find_if(dumps.begin(), dumps.end(), [&](const DumpDesc &d) { return d.id == w[1]; });
+            if (it == dumps.end()) fail(MDPD_E_ARG, "Could not find undump ID %s", w[1].c_str());
+            dumps.erase(it);
+        } else if (c == "dump_modify") {
         } else if (c == "thermo") {
             need(2);
             thermo_every = (int)parse_int(w[1]);
@@ -500,6 +637,92 @@ struct mdpd_system {
             fprintf(f, "%d %.16e %.16e %.16e\n", h_tag[i], h_v[3 * i], h_v[3 * i + 1], h_v[3 * i + 2]);
         const bool ok = fclose(f) == 0;
         if (!ok) fail(MDPD_E_ARG, "error writing data file %s", path.c_str());
+    }
+
+    void cmd_dump(const std::vector<std::string> &w)
+    {
+        if (w.size() < 7) fail(MDPD_E_ARG, "Illegal dump command");
+        if (w[3] != "custom") fail(MDPD_E_UNSUPPORTED, "dump style %s is not supported (custom only)", w[3].c_str());
+        DumpDesc d;
+        d.id = w[1];
+        d.groupbit = need_group(w[2]);
+        d.every = parse_int(w[4]);
+        if (d.every <= 0) fail(MDPD_E_ARG, "Illegal dump command");
+        d.file = w[5];
+        static const char *known[] = {"id", "type", "x", "y", "z", "xs", "ys", "zs", "xu", "yu", "zu",
+                                      "ix", "iy", "iz", "vx", "vy", "vz", "fx", "fy", "fz"};
+        for (size_t k = 6; k < w.size(); ++k) {
+            bool ok = false;
+            for (const char *q : known) ok |= w[k] == q;
+            if (!ok) fail(MDPD_E_UNSUPPORTED, "dump custom column %s is not supported", w[k].c_str());
+            d.cols.push_back(w[k]);
+        }
+        for (auto &e : dumps)
+            if (e.id == d.id) fail(MDPD_E_ARG, "Reuse of dump ID %s", d.id.c_str());
+        dumps.push_back(d);
+    }
+
+    bool dump_due(long long t) const
+    {
+        for (auto &d : dumps)
+            if (t % d.every == 0) return true;
+        return false;
+    }
+
+    // the state of timestep t to every dump due at t.  `prev`: right after a
+    // fused pass with output (x(t) in the buffer the pass read, v(t) and f(t)
+    // written by the pass); otherwise the current buffer (setup, end of run).
+    void write_dumps(long long t, bool prev)
+    {
+        const int n = (int)h_tag.size();
+        sync_sys();
+        const int sel = prev ? hsys.sel ^ 1 : hsys.sel, sela = hsys.sela;
+        std::vector<int> org = download(origin[sela].p, cap);
+        std::vector<int> im = download(image[sela].p, cap);
+        std::vector<int> mk = download(mask[sela].p, cap);
+        std::vector<int> tg = download(tag[sela].p, cap);
+        std::vector<float4> px = download(pos[sel].p, cap);
+        std::vector<float4> pv = download(prev ? vend.p : vel[sel].p, cap);
+        std::vector<float4> pf = download(frc.p, cap);
+        hip_check(hipStreamSynchronize(stream), "dump");
+        std::vector<int> slot(n, -1);
+        for (int k = 0; k < n; ++k) slot[org[k]] = k;
+        for (auto &d : dumps) {
+            if (t % d.every != 0) continue;
+            FILE *f = fopen(d.file.c_str(), d.started ? "a" : "w");
+            if (!f) fail(MDPD_E_ARG, "Cannot open dump file %s", d.file.c_str());
+            d.started = true;
+            int cnt = 0;
+            for (int k = 0; k < n; ++k) cnt += (mk[k] & d.groupbit) != 0;
+            fprintf(f, "ITEM: TIMESTEP\n%lld\nITEM: NUMBER OF ATOMS\n%d\nITEM: BOX BOUNDS pp pp pp\n", t, cnt);
+            for (int a = 0; a < 3; ++a) fprintf(f, "%.16e %.16e\n", lo[a], hi[a]);
+            fprintf(f, "ITEM: ATOMS");
+            for (auto &c : d.cols) fprintf(f, " %s", c.c_str());
+            fprintf(f, "\n");
+            for (int i = 0; i < n; ++i) {
+                const int k = slot[i];
+                if (k < 0 || !(mk[k] & d.groupbit)) continue;
+                const float xv[3] = {px[k].x, px[k].y, px[k].z};
+                const int iv[3] = {(im[k] & 1023) - 512, ((im[k] >> 10) & 1023) - 512, ((im[k] >> 20) & 1023) - 512};
+                for (size_t q = 0; q < d.cols.size(); ++q) {
+                    const std::string &c = d.cols[q];
+                    const char *sp = q ? " " : "";
+                    const int a = c.back() == 'x' ? 0 : (c.back() == 'y' ? 1 : 2);
+                    if (c == "id") fprintf(f, "%s%d", sp, tg[k]);
+                    else if (c == "type") { int ty; memcpy(&ty, &px[k].w, 4); fprintf(f, "%s%d", sp, ty + 1); }
+                    else if (c == "x" || c == "y" || c == "z") fprintf(f, "%s%.9g", sp, xv[a]);
+                    else if (c == "xs" || c == "ys" || c == "zs") fprintf(f, "%s%.9g", sp, (xv[c[0] == 'x' ? 0 : (c[0] == 'y' ? 1 : 2)] - lo[c[0] == 'x' ? 0 : (c[0] == 'y' ? 1 : 2)]) / (hi[c[0] == 'x' ? 0 : (c[0] == 'y' ? 1 : 2)] - lo[c[0] == 'x' ? 0 : (c[0] == 'y' ? 1 : 2)]));
+                    else if (c == "xu" || c == "yu" || c == "zu") {
+                        const int b = c[0] == 'x' ? 0 : (c[0] == 'y' ? 1 : 2);
+                        fprintf(f, "%s%.9g", sp, xv[b] + iv[b] * (hi[b] - lo[b]));
+                    } else if (c == "ix" || c == "iy" || c == "iz") fprintf(f, "%s%d", sp, iv[c[1] == 'x' ? 0 : (c[1] == 'y' ? 1 : 2)]);
+                    else if (c == "vx" || c == "vy" || c == "vz") fprintf(f, "%s%.9g", sp, c[1] == 'x' ? pv[k].x : (c[1] == 'y' ? pv[k].y : pv[k].z));
+                    else fprintf(f, "%s%.9g", sp, c[1] == 'x' ? pf[k].x : (c[1] == 'y' ? pf[k].y : pf[k].z));
+                }
+                fprintf(f, "\n");
+            }
+            fclose(f);
+        }
     }
 
     // velocity G set vx vy vz (NULL keeps a component); src/velocity.cpp
@@ -1236,6 +1459,7 @@ struct mdpd_system {
         if (nsteps < 0) fail(MDPD_E_ARG, "invalid run length");
         if (!run_style_ok) fail(MDPD_E_STATE, "run_style mvv/meso must be set for the USER-MESO styles");
         if (!setup_done || dirty) setup();
+        if (!dumps.empty() && dump_due(ntimestep)) write_dumps(ntimestep, false);   // Verlet::setup output
         if (nsteps == 0) return;
         PoolDev P = view();
         timed(PH_INITIAL, [&]() { hip_check(launch_initial(P, stream), "initial"); });
@@ -1254,7 +1478,9 @@ struct mdpd_system {
                 rebuild();
                 ago = 0;
             }
-            force_pass(k < nsteps ? MODE_FUSED : MODE_FINAL, ntimestep + k);
+            const bool out = !dumps.empty() && dump_due(ntimestep + k);
+            force_pass(k < nsteps ? (out ? MODE_FUSED_OUT : MODE_FUSED) : MODE_FINAL, ntimestep + k);
+            if (out) write_dumps(ntimestep + k, k < nsteps);
         }
         hip_check(hipStreamSynchronize(stream), "run");
         collect_events();
