@@ -419,7 +419,7 @@ def test_dump_custom_frames_match_the_state_of_their_step(engine, tmp_path):
     assert sorted(fr) == [0, 5, 10]
     assert np.array_equal(fr[0]["id"], np.arange(1, n + 1))
     # step 0: the setup state; step 10: the end of the run
-    np.testing.assert_array_equal(np.c_[fr[0]["x"], fr[0]["y"], fr[0]["z"]], x.astype(np.float32))
+    np.testing.assert_allclose(np.c_[fr[0]["x"], fr[0]["y"], fr[0]["z"]], x, rtol=1e-7, atol=1e-7)
     for key, name in (("x", "x"), ("vx", "v"), ("fx", "f")):
         got = np.c_[fr[10][key], fr[10][key.replace("x", "y", 1) if key != "x" else "y"],
                     fr[10][key.replace("x", "z", 1) if key != "x" else "z"]]
