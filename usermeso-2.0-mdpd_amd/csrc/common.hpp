@@ -69,7 +69,6 @@ struct SysDev {
     uint32_t seed_cur;           // premix_TEA<64>(seed, step): signatures of a fresh system
     int phase;                   // pooled engine: PH_FRESH / PH_RUN / PH_LAST / PH_DONE
     int job;                     // pooled engine: caller's job id of this slot
-    int nne;                     // non-empty cells at the last re-sort
     int split_last;              // pooled engine: waves per tile of the system's last pass
     // arbitrary boundary: wall-scaled gamma / sigma of mobile-wall pairs
     // (pair_style mdpd/meso ... mobile wall) and wall fraction + bounce-back
@@ -95,7 +94,7 @@ struct PoolDev {
     float4 *vend;        // end-of-step velocity  (output steps only)
     // sort / cell scratch
     int *pcell;          // global cell slot per particle
-    int *pslot;          // arrival rank within the cell
+    int *pslot;          // arrival rank within the cell; after the re-sort: cell of each sorted slot
     int *psub;           // 12-bit sub-bin Morton code
     int *tmp;            // particle index per sorted slot (pre-order)
     unsigned long long *key;  // optional: reference sort key per particle (debug)
@@ -111,9 +110,7 @@ struct PoolDev {
     int *wave_sys;       // system of each 64-particle wave
     int *moved;          // per 64-particle tile: a particle moved past skin/2 this step
     int *status;         // pooled engine, per system: job << 3 | 1 stale | 2 done | 4 overflow
-    int *cl;             // per particle slot: a system's non-empty cells (staging)
-    int *wl;             // rebuild work list: non-empty cell slots of the systems being rebuilt
-    int *work_total;     // entries in wl
+    int *cl;             // per particle slot, in cell-segment order: sub-bin key (re-sort scratch)
     SysDev *sys;
     float *coef;         // pooled coefficient tables
     float *minv;         // pooled per-type 1/m

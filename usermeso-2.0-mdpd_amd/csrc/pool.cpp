@@ -141,7 +141,7 @@ struct Job {
 
 // bytes of device memory per particle slot (state, scratch, 2 cell slots)
 // and per neighbor-row entry (nlist + clist)
-constexpr double BYTES_PER_SLOT = 264.0;
+constexpr double BYTES_PER_SLOT = 256.0;
 constexpr double BYTES_PER_ENTRY = 8.0;
 constexpr int CELLS_PER_SLOT = 2;
 
@@ -162,7 +162,7 @@ struct mdpd_pool {
     DevBuf<int> tag[2], mask[2], origin[2], image[2];
     DevBuf<float> rho, vir, coef, minv, mmass, phi[2];
     DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys;
-    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl, wl, work_total;
+    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl;
     DevBuf<SysDev> dsys;
     RangeAlloc parts, cells;              // parts in units of 256 slots
     std::vector<int> free_slots, slot_job;
@@ -273,7 +273,7 @@ struct mdpd_pool {
         P.cell_rank = cell_rank.p; P.cell_sys = cell_sys.p;
         P.nlist = nlist.p; P.ncount = ncount.p; P.wave_sys = wave_sys.p;
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
-        P.cl = cl.p; P.wl = wl.p; P.work_total = work_total.p;
+        P.cl = cl.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
         P.cap = cap; P.nact = nact; P.ncellslots = nslots; P.nsys = max_sys; P.maxnbr = maxnbr;
         return P;
@@ -364,7 +364,7 @@ struct mdpd_pool {
         nlist.alloc((size_t)cap * maxnbr);
         clist.alloc((size_t)cap * maxnbr);
         wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE); status.alloc(max_sys);
-        cl.alloc(cap); wl.alloc(cap); work_total.alloc(1);
+        cl.alloc(cap);
         cell_count.alloc(nslots); cell_start.alloc(nslots); cell_xyz.alloc(nslots);
         cell_rank.alloc(nslots); cell_sys.alloc(nslots);
         coef.alloc((size_t)max_sys * MAXTYPES * MAXTYPES * NCOEF);
@@ -373,7 +373,6 @@ struct mdpd_pool {
         hip_check(hipMemsetAsync(cell_count.p, 0, sizeof(int) * nslots, stream), "memset");
         hip_check(hipMemsetAsync(cell_sys.p, 0xFF, sizeof(int) * nslots, stream), "memset");
         hip_check(hipMemsetAsync(wave_sys.p, 0xFF, sizeof(int) * (cap / WAVE), stream), "memset");
-        hip_check(hipMemsetAsync(work_total.p, 0, sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(status.p, 0, sizeof(int) * max_sys, stream), "memset");
         hip_check(hipMemsetAsync(dsys.p, 0, sizeof(SysDev) * max_sys, stream), "memset");
         hip_check(hipMemsetAsync(ncount.p, 0, sizeof(int) * cap, stream), "memset");
@@ -404,10 +403,8 @@ struct mdpd_pool {
             graph_valid = false;
             ++regrows;
             PoolDev P = view();
-            hip_check(hipMemsetAsync(work_total.p, 0, sizeof(int), stream), "memset");
-            hip_check(launch_relist(P, stream), "relist");
-            hip_check(hipMemsetAsync(work_total.p, 0, sizeof(int), stream), "memset");
-            std::vector<SysDev> h(max_sys);
+                hip_check(launch_relist(P, stream), "relist");
+                std::vector<SysDev> h(max_sys);
             hip_check(hipMemcpyAsync(h.data(), dsys.p, sizeof(SysDev) * max_sys, hipMemcpyDeviceToHost, stream),
                       "readback");
             hip_check(hipStreamSynchronize(stream), "relist");
@@ -462,7 +459,6 @@ struct mdpd_pool {
         d.job = j.id;
         d.sel = 0;
         d.sela = 0;
-        d.nne = 0;
         d.split_last = split;
         const int npad = j.npad, nc1 = j.ncell + 1, ntw = npad / WAVE;
         const size_t ncoef = j.ds.coef.size(), nt = j.ntypes;

@@ -217,7 +217,7 @@ struct mdpd_system {
     DevBuf<int> tag[2], mask[2], origin[2], image[2];
     DevBuf<float> rho, vir, coef, minv, mmass, phi[2];
     DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys;
-    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl, wl, work_total;
+    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl;
     DevBuf<unsigned long long> key;
     DevBuf<SysDev> dsys;
     int *pinned = nullptr;            // host-visible readback of flags
@@ -1130,7 +1130,7 @@ struct mdpd_system {
         P.cell_rank = cell_rank.p; P.cell_sys = cell_sys.p;
         P.nlist = nlist.p; P.ncount = ncount.p; P.wave_sys = wave_sys.p;
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
-        P.cl = cl.p; P.wl = wl.p; P.work_total = work_total.p;
+        P.cl = cl.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
         P.cap = cap; P.nact = cap; P.ncellslots = nslots; P.nsys = 1; P.maxnbr = maxnbr;
         return P;
@@ -1320,7 +1320,7 @@ struct mdpd_system {
         nlist.alloc((size_t)cap * maxnbr);
         clist.alloc((size_t)cap * maxnbr);
         wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE); status.alloc(1);
-        cl.alloc(cap); wl.alloc(cap); work_total.alloc(1);
+        cl.alloc(cap);
         cell_count.alloc(nslots); cell_start.alloc(nslots); cell_xyz.alloc(nslots);
         cell_rank.alloc(nslots); cell_sys.alloc(nslots);
         coef.alloc(ds.coef.size()); minv.alloc(ntypes); mmass.alloc(ntypes); dsys.alloc(1);
@@ -1347,7 +1347,6 @@ struct mdpd_system {
         up(minv.p, ds.minv.data(), ntypes * sizeof(float));
         up(mmass.p, ds.mass.data(), ntypes * sizeof(float));
         hip_check(hipMemsetAsync(cell_count.p, 0, nslots * sizeof(int), stream), "memset");
-        hip_check(hipMemsetAsync(work_total.p, 0, sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(ncount.p, 0, cap * sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(rho.p, 0, cap * sizeof(float), stream), "memset");
         hip_check(hipMemsetAsync(phi[0].p, 0, cap * sizeof(float), stream), "memset");
