@@ -152,6 +152,46 @@ def test_neighbor_sets_bit_exact(setup_case, orc):
         assert got[i] == ref, i
 
 
+def _assert_stencil_order(xw, counts, rows, L, cut):
+    """Each row lists the 27 stencil cells in order k = (ox+1) + 3(oy+1) +
+    9(oz+1) and, inside a cell, sorted indices ascending: the order of a row
+    built over whole cells, whatever sub-cells the build skipped."""
+    nb = int(L * (1.0 / cut))
+    bininv = 1.0 / (L / nb)
+    cell = np.clip((xw.astype(np.float64) * bininv + 1.0).astype(np.int64), 1, nb) - 1
+    n = len(xw)
+    for i in range(n):
+        e = rows[i, :counts[i]].astype(np.int64) & 0xFFFFFFFF
+        j, code = e & IDX, e >> 27
+        shift = np.stack([code % 3 - 1, code // 3 % 3 - 1, code // 9 - 1], axis=1)
+        off = cell[j] + shift * nb - cell[i]
+        assert np.all(np.abs(off) <= 1), i
+        k = (off[:, 0] + 1) + 3 * (off[:, 1] + 1) + 9 * (off[:, 2] + 1)
+        key = k * n + j
+        assert np.all(np.diff(key) > 0), i
+
+
+@pytest.mark.parametrize("L", [8.0, 6.5])
+def test_neighbor_rows_in_stencil_order(engine, orc, L):
+    # L = 8: bins of 4/3 > rc + skin, the build skips far sub-cells;
+    # L = 6.5: bins of exactly rc + skin, every sub-cell is visited
+    x, v = uniform_box(int(6.05 * L**3), L, 7)
+    bs = L / int(L / 1.3)
+    x[:40, 1] = (np.arange(40) % (2 * int(L / 1.3))) * bs / 2     # on half-cell boundaries
+    x[40:80, 2] = (np.arange(40) % (2 * int(L / 1.3))) * bs / 2
+    s = mdpd_system(engine, x, v, L)
+    s.setup()
+    origin, _, xw = s.debug_sorted()
+    counts, rows = s.debug_neighbors()
+    got = _neighbor_sets(origin, counts, rows)
+    xc = np.empty_like(xw)
+    xc[origin] = xw
+    _, rrows = orc.neighbors(xc, (0, 0, 0), (L, L, L), 1.3)
+    for i in range(len(x)):
+        assert got[i] == {(int(e) & IDX, int(e) >> 27) for e in rrows[i]}, i
+    _assert_stencil_order(xw, counts, rows, L, 1.3)
+
+
 def test_density_parity(setup_case, orc):
     s, L, x, v = setup_case
     xc = s.gather("x")

@@ -15,6 +15,40 @@ constexpr int IMG_SHIFT = 27;      // neighbor entry = j | image << 27
 constexpr uint32_t IDX_MASK = (1u << IMG_SHIFT) - 1u;
 constexpr int IMG_ZERO = 13;       // (0,0,0) image: (sx+1) + 3(sy+1) + 9(sz+1)
 
+// Neighbor rows are stored per 64-particle tile in chunks of NBR_CHUNK
+// entries per particle: entry q of tile lane l is at
+//   tile * maxnbr * 64 + (q / 8) * 512 + l * 8 + q % 8.
+// Consecutive entries of one particle fill whole 32-byte sectors (the row
+// builder appends a run of a particle's entries with one store), and a
+// reader loads four entries of its particle with one 16-byte load.  maxnbr
+// is always a multiple of 16.
+constexpr int NBR_CHUNK = 8;
+
+__host__ __device__ inline size_t nbr_at(int i, int q, int maxnbr)
+{
+    return (size_t)(i >> 6) * maxnbr * WAVE + (size_t)(q & ~(NBR_CHUNK - 1)) * WAVE +
+           (size_t)(i & 63) * NBR_CHUNK + (q & (NBR_CHUNK - 1));
+}
+
+// Each cell is split into SUBCELLS sub-cells by the top bits of the sub-bin
+// Morton key (bit 10: upper half in y, bit 11: upper half in z).  The sort
+// key orders a cell's particles by that code first, so every sub-cell is a
+// contiguous segment of the cell and the counting sort runs per sub-cell
+// (fewer same-address atomics, a short rank loop per particle).
+constexpr int SUBCELLS = 4;
+constexpr int SUBCELL_SHIFT = 10;
+
+// The hardware dispatches workgroup b to XCD b % 8 (each XCD has its own
+// L2); xcd_logical gives XCD x the contiguous logical range
+// [x*per, (x+1)*per) of a grid of xcd_grid(n) workgroups.
+constexpr int NXCD = 8;
+__device__ inline int xcd_logical(int b, int nlogical)
+{
+    const int per = (nlogical + NXCD - 1) / NXCD;
+    return (b % NXCD) * per + b / NXCD;
+}
+static inline int xcd_grid(int nlogical) { return ((nlogical + NXCD - 1) / NXCD) * NXCD; }
+
 enum Style : int { STYLE_MDPD = 0, STYLE_DPD = 1 };
 
 // Lifecycle of a system in a pooled (batched / served) launch.  A job of K
@@ -93,13 +127,13 @@ struct PoolDev {
     float *vir;          // 6 * cap, SoA blocks  (output steps only)
     float4 *vend;        // end-of-step velocity  (output steps only)
     // sort / cell scratch
-    int *pcell;          // global cell slot per particle
-    int *pslot;          // arrival rank within the cell; after the re-sort: cell of each sorted slot
+    int *pcell;          // global sub-cell slot per particle (cell slot * SUBCELLS + sub-cell)
+    int *pslot;          // arrival rank within the sub-cell; after the re-sort: sub-cell slot of each sorted slot
     int *psub;           // 12-bit sub-bin Morton code
     int *tmp;            // particle index per sorted slot (pre-order)
     unsigned long long *key;  // optional: reference sort key per particle (debug)
-    int *cell_count;     // per cell slot
-    int *cell_start;     // per cell slot, global particle index; slot ncell = end
+    int *cell_count;     // per sub-cell slot
+    int *cell_start;     // per sub-cell slot, global particle index; slot ncell * SUBCELLS = end
     int *cell_xyz;       // packed interior coords of the cell at this Morton rank
     int *cell_rank;      // Morton rank of linear cell id (per system, cell_base + lin)
     int *cell_sys;       // system of each cell slot
@@ -118,6 +152,7 @@ struct PoolDev {
     int cap;             // particle slots (multiple of 64): array extent and strides
     int nact;            // slots [0, nact) hold systems: extent of the launches (<= cap)
     int ncellslots;
+    int ncact;           // cell slots [0, ncact) hold systems: extent of per-cell launches
     int nsys;
     int maxnbr;
 };

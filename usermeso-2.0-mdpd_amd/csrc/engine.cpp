@@ -287,7 +287,7 @@ int mdpd_debug_neighbors(mdpd_system *sys, int *maxnbr, int *counts, int *rows)
             for (int i = 0; i < n; ++i) {
                 const int cnt = std::min(c[i], m);
                 for (int p = 0; p < m; ++p)
-                    rows[(size_t)i * m + p] = p < cnt ? h[(size_t)(i >> 6) * m * WAVE + (size_t)p * WAVE + (i & 63)] : -1;
+                    rows[(size_t)i * m + p] = p < cnt ? h[mdpd::nbr_at(i, p, m)] : -1;
             }
         }
     });

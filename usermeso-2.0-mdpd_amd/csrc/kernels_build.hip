@@ -17,8 +17,9 @@ namespace mdpd {
 #pragma clang fp contract(off)
 
 // --------------------------------------------------------------------------
-// K1: wrap into the box (Domain::pbc in FP32), reference sort key, cell slot,
-// arrival rank inside the cell.
+// K1: wrap into the box (Domain::pbc in FP32), reference sort key, sub-cell
+// slot (cell slot * SUBCELLS + top bits of the sub-bin key), arrival rank
+// inside the sub-cell.
 // --------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_keys(PoolDev P)
 {
@@ -70,16 +71,24 @@ __global__ void __launch_bounds__(256) k_keys(PoolDev P)
         P.key[i] = (z1 << d->l1_shift) | z2;
     }
     const int lin = (b[0] - 1) + d->nb[0] * ((b[1] - 1) + d->nb[1] * (b[2] - 1));
-    const int gc = d->cell_base + P.cell_rank[d->cell_base + lin];
+    const int gc = (d->cell_base + P.cell_rank[d->cell_base + lin]) * SUBCELLS + (int)(z2 >> SUBCELL_SHIFT);
     P.pcell[i] = gc;
     P.psub[i] = (int)z2;
     P.pslot[i] = atomicAdd(P.cell_count + gc, 1);
 }
 
 // --------------------------------------------------------------------------
-// K2: exclusive scan of cell counts in Morton-rank order, one workgroup per
-// system; clears the counts for the next rebuild.
+// K2: exclusive scan of sub-cell counts in (cell Morton rank, sub-cell)
+// order, per system, from its first particle slot; clears the counts for the
+// next rebuild.  Chunks of SCAN_CH counts, three launches without waits:
+//   k_scan_sums   per chunk: its total, parked in the chunk's first start slot
+//   k_scan_chunks per system: exclusive scan of the chunk totals in place,
+//                 and the end entry
+//   k_scan_apply  per chunk: exclusive scan of its counts from its offset
+// Grid (G, nsys): workgroup g of a system takes chunks g, g + G, ...
 // --------------------------------------------------------------------------
+constexpr int SCAN_BLK = 256, SCAN_PER = 16, SCAN_CH = SCAN_BLK * SCAN_PER;
+
 __device__ inline int wave_incl_scan(int x, int lane)
 {
     for (int off = 1; off < 64; off <<= 1) {
@@ -89,57 +98,93 @@ __device__ inline int wave_incl_scan(int x, int lane)
     return x;
 }
 
-__global__ void __launch_bounds__(1024) k_scan(PoolDev P)
+// exclusive scan of one value per thread over the workgroup; returns the
+// workgroup total through *total
+__device__ inline int block_excl_scan(int x, int *wsum, int *total)
 {
-    const int s = blockIdx.x;
-    SysDev *d = P.sys + s;
-    if (!d->active || !d->stale) return;
-    // chunks of 8192 counts: coalesced load into LDS, 8 consecutive counts per
-    // thread, wave scan + scan of the 16 wave totals, coalesced store
-    constexpr int PER = 8, CH = 1024 * PER;
-    __shared__ int buf[CH];
-    __shared__ int wex[16];
-    __shared__ int carry;
-    const int nc = d->ncell, base = d->cell_base, pb = d->part_base;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    if (t == 0) carry = pb;
+    const int incl = wave_incl_scan(x, lane);
+    if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    for (int c0 = 0; c0 < nc; c0 += CH) {
-        for (int k = t; k < CH; k += 1024) {
-            const int c = c0 + k;
+    int before = 0, all = 0;
+    for (int k = 0; k < SCAN_BLK / 64; ++k) {
+        const int y = wsum[k];
+        before += k < w ? y : 0;
+        all += y;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - x;
+}
+
+__global__ void __launch_bounds__(SCAN_BLK) k_scan_sums(PoolDev P)
+{
+    const SysDev *d = P.sys + blockIdx.y;
+    if (!d->active || !d->stale) return;
+    __shared__ int wsum[SCAN_BLK / 64];
+    const int nc = d->ncell * SUBCELLS, base = d->cell_base * SUBCELLS;
+    for (int c0 = blockIdx.x * SCAN_CH; c0 < nc; c0 += gridDim.x * SCAN_CH) {
+        int sum = 0;
+        for (int k = threadIdx.x; k < SCAN_CH; k += SCAN_BLK)
+            if (c0 + k < nc) sum += P.cell_count[base + c0 + k];
+        int total;
+        block_excl_scan(sum, wsum, &total);
+        if (threadIdx.x == 0) P.cell_start[base + c0] = total;
+    }
+}
+
+__global__ void __launch_bounds__(SCAN_BLK) k_scan_chunks(PoolDev P)
+{
+    const SysDev *d = P.sys + blockIdx.x;
+    if (!d->active || !d->stale) return;
+    __shared__ int wsum[SCAN_BLK / 64];
+    const int nc = d->ncell * SUBCELLS, base = d->cell_base * SUBCELLS;
+    const int nch = (nc + SCAN_CH - 1) / SCAN_CH;
+    int carry = d->part_base;
+    for (int k0 = 0; k0 < nch; k0 += SCAN_BLK) {
+        const int k = k0 + threadIdx.x;
+        const int v = k < nch ? P.cell_start[base + k * SCAN_CH] : 0;
+        int total;
+        const int ex = block_excl_scan(v, wsum, &total);
+        if (k < nch) P.cell_start[base + k * SCAN_CH] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) P.cell_start[base + nc] = d->part_base + d->n;
+}
+
+__global__ void __launch_bounds__(SCAN_BLK) k_scan_apply(PoolDev P)
+{
+    const SysDev *d = P.sys + blockIdx.y;
+    if (!d->active || !d->stale) return;
+    __shared__ int buf[SCAN_CH];
+    __shared__ int wsum[SCAN_BLK / 64];
+    const int nc = d->ncell * SUBCELLS, base = d->cell_base * SUBCELLS;
+    const int t = threadIdx.x;
+    for (int c0 = blockIdx.x * SCAN_CH; c0 < nc; c0 += gridDim.x * SCAN_CH) {
+        const int off = P.cell_start[base + c0];     // the chunk's offset
+        // coalesced load into LDS (counts cleared), SCAN_PER consecutive
+        // counts per thread, workgroup scan, coalesced store
+        for (int k = t; k < SCAN_CH; k += SCAN_BLK) {
             int v = 0;
-            if (c < nc) { v = P.cell_count[base + c]; P.cell_count[base + c] = 0; }
+            if (c0 + k < nc) { v = P.cell_count[base + c0 + k]; P.cell_count[base + c0 + k] = 0; }
             buf[k] = v;
         }
         __syncthreads();
-        int v[PER], sum = 0;
-        for (int k = 0; k < PER; ++k) { v[k] = buf[t * PER + k]; sum += v[k]; }
-        const int incl = wave_incl_scan(sum, lane);
-        if (lane == 63) wex[w] = incl;
+        int v[SCAN_PER], sum = 0;
+        for (int k = 0; k < SCAN_PER; ++k) { v[k] = buf[t * SCAN_PER + k]; sum += v[k]; }
+        int total;
+        int run = off + block_excl_scan(sum, wsum, &total);
+        for (int k = 0; k < SCAN_PER; ++k) { buf[t * SCAN_PER + k] = run; run += v[k]; }
         __syncthreads();
-        if (t == 0) {
-            int acc = 0;
-            for (int k = 0; k < 16; ++k) { const int y = wex[k]; wex[k] = acc; acc += y; }
-        }
-        __syncthreads();
-        int run = carry + wex[w] + incl - sum;
-        for (int k = 0; k < PER; ++k) { buf[t * PER + k] = run; run += v[k]; }
-        __syncthreads();
-        for (int k = t; k < CH; k += 1024) {
-            const int c = c0 + k;
-            if (c < nc) P.cell_start[base + c] = buf[k];
-        }
-        if (t == 1023) carry = run;
+        for (int k = t; k < SCAN_CH; k += SCAN_BLK)
+            if (c0 + k < nc) P.cell_start[base + c0 + k] = buf[k];
         __syncthreads();
     }
-    if (t == 0) P.cell_start[base + nc] = pb + d->n;
 }
 
 // --------------------------------------------------------------------------
-// K3: scatter particle indices into their cell segments (arrival order).
-// --------------------------------------------------------------------------
-// K3: scatter particle indices into their cell segments (arrival order), the
-// sub-bin key beside each (cl is free once k_scan has used it).
+// K3: scatter particle indices into their sub-cell segments (arrival order),
+// the sub-bin key beside each (cl is free once k_scan has used it).
 // --------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_scatter(PoolDev P)
 {
@@ -156,11 +201,12 @@ __global__ void __launch_bounds__(256) k_scatter(PoolDev P)
 }
 
 // --------------------------------------------------------------------------
-// K4: one thread per particle, by its arrival slot f in its cell's segment:
-// its rank in the cell by (sub-bin key, previous index) -- the stable sort of
-// the 2-level Morton key -- gives its sorted slot; every per-particle array
-// moves there in the other buffer, and the sorted slot records its cell (in
-// pslot, free after k_scatter) for the neighbor-row build.
+// K4: one thread per particle, by its arrival slot f in its sub-cell's
+// segment: its rank in the sub-cell by (sub-bin key, previous index) -- with
+// the sub-cells in key order this is the stable sort of the 2-level Morton
+// key -- gives its sorted slot; every per-particle array moves there in the
+// other buffer, and the sorted slot records its sub-cell (in pslot, free after
+// k_scatter) for the neighbor-row build.
 // --------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 {
@@ -204,83 +250,126 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 }
 
 // --------------------------------------------------------------------------
-// K5: one lane per centre particle (sorted order): its row lists every
+// K5: neighbor rows, one wave per cell.  A centre's row lists every
 // (j, image) of the 27 stencil cells of its cell -- stencil cell k has offset
 // (k%3-1, k/3%3-1, k/9-1), periodic images shift by +-L -- in stencil order,
 // then j ascending, with (float) r^2 <= (float)(rc+skin)^2 evaluated without
 // contraction (neigh_build_meso.cu:86-93), itself excluded.  The set is the
-// reference's (neigh_build_meso.cu:58-118); each lane appends to its own row:
-// entry p of particle i at tile*maxnbr*64 + p*64 + i%64.
+// reference's (neigh_build_meso.cu:58-118).
+//
+// All centres of a cell share the stencil, so the wave walks the stencil's
+// candidates 64 at a time, one per lane (coalesced loads, image shift applied
+// once), NL_ITER iterations held in registers, and tests them against one
+// centre after another (the centre's position is uniform).  The lanes that
+// pass append in lane order (ballot + mbcnt), which keeps the row order.
 // relist: rows of the current, already sorted buffer (the row width grew);
 // otherwise of the data k_sortgather just wrote.
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_nlist(PoolDev P, int relist)
+constexpr int NL_ITER = 8;
+
+__global__ void __launch_bounds__(WAVE) k_nlist(PoolDev P, int relist, int ncells)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.nact) return;
-    const int s = P.wave_sys[i >> 6];
+    const int cs = xcd_logical(blockIdx.x, ncells);        // cell slot (Morton rank order)
+    if (cs >= ncells) return;
+    const int s = P.cell_sys[cs];
     if (s < 0) return;
     SysDev *d = P.sys + s;
     if (!d->active || (relist ? d->stale != 0 : d->stale == 0)) return;
-    if (i >= d->part_base + d->n) return;
+    const int cb = d->cell_base;
+    if (cs < cb || cs >= cb + d->ncell) return;           // the end slot holds no cell
+    const int c0 = P.cell_start[cs * SUBCELLS], c1 = P.cell_start[(cs + 1) * SUBCELLS];
+    if (c0 >= c1) return;
+    const int lane = threadIdx.x;
     const float4 *pos = P.pos[relist ? d->sel : d->sel ^ 1];
-    const int xyz = P.cell_xyz[P.pslot[i]];
-    const int cx = xyz & 1023, cy = (xyz >> 10) & 1023, cz = (xyz >> 20) & 1023;
-    const int nb0 = d->nb[0], nb1 = d->nb[1], nb2 = d->nb[2], cb = d->cell_base;
-    const float4 pi = pos[i];
+    // stencil table: lane k < 27 holds cell k's first particle, its length,
+    // image code and shift
+    int kst = 0, klen = 0;
+    uint32_t kcode = IMG_ZERO;
+    float ksx = 0.f, ksy = 0.f, ksz = 0.f;
+    if (lane < 27) {
+        const int xyz = P.cell_xyz[cs];
+        const int cx = xyz & 1023, cy = (xyz >> 10) & 1023, cz = (xyz >> 20) & 1023;
+        const int nb0 = d->nb[0], nb1 = d->nb[1], nb2 = d->nb[2];
+        int nx = cx + lane % 3 - 1, ny = cy + lane / 3 % 3 - 1, nz = cz + lane / 9 - 1;
+        int sx = 0, sy = 0, sz = 0;
+        if (nx < 0) { nx += nb0; sx = -1; } else if (nx >= nb0) { nx -= nb0; sx = 1; }
+        if (ny < 0) { ny += nb1; sy = -1; } else if (ny >= nb1) { ny -= nb1; sy = 1; }
+        if (nz < 0) { nz += nb2; sz = -1; } else if (nz >= nb2) { nz -= nb2; sz = 1; }
+        const int rank = P.cell_rank[cb + nx + nb0 * (ny + nb1 * nz)];
+        kst = P.cell_start[(cb + rank) * SUBCELLS];
+        klen = P.cell_start[(cb + rank + 1) * SUBCELLS] - kst;
+        kcode = (uint32_t)((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
+        ksx = (float)sx * d->L[0];
+        ksy = (float)sy * d->L[1];
+        ksz = (float)sz * d->L[2];
+    }
+    const int kend = wave_incl_scan(klen, lane);
+    const int kpre = kend - klen;                           // candidates before cell k
+    const int ntot = __builtin_amdgcn_readlane(kend, 26);
     const float rc2 = d->rc2_list;
     const int maxnbr = P.maxnbr;
-    int *row = P.nlist + (size_t)(i >> 6) * maxnbr * WAVE + (i & 63);
-    int cnt = 0;
-    auto test = [&](int j, const float4 &q, float shx, float shy, float shz, uint32_t code) {
-        const float xj = q.x + shx, yj = q.y + shy, zj = q.z + shz;
-        const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-        const float r2 = (dx * dx + dy * dy) + dz * dz;
-        if (r2 <= rc2 && !(j == i && code == (uint32_t)IMG_ZERO)) {
-            if (cnt < maxnbr) row[(size_t)cnt * WAVE] = (int)((uint32_t)j | (code << IMG_SHIFT));
-            ++cnt;
-        }
-    };
-    // one plane of the stencil (9 cells, fixed dz) at a time: the 9 rank
-    // look-ups, then the 18 offset look-ups, are issued together
-    for (int kz = 0; kz < 3; ++kz) {
-        int nz = cz + kz - 1, sz = 0;
-        if (nz < 0) { nz += nb2; sz = -1; } else if (nz >= nb2) { nz -= nb2; sz = 1; }
-        int ns[9], sxa[9], sya[9];
+    for (int cg = c0; cg < c1; cg += WAVE) {
+        const int ng = min(WAVE, c1 - cg);
+        int cntv = 0;                                       // lane l: row length of centre cg + l
+        const float4 pg = pos[min(cg + lane, c1 - 1)];      // lane l: position of centre cg + l
+        for (int tb = 0; tb < ntot; tb += NL_ITER * WAVE) {
+            // candidates tb + 64 m + lane: stencil cell by binary search over
+            // the exclusive prefix (the last cell starting at or before t)
+            float qx[NL_ITER], qy[NL_ITER], qz[NL_ITER];
+            int qe[NL_ITER], qself[NL_ITER];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            int nx = cx + q % 3 - 1, ny = cy + q / 3 - 1, sx = 0, sy = 0;
-            if (nx < 0) { nx += nb0; sx = -1; } else if (nx >= nb0) { nx -= nb0; sx = 1; }
-            if (ny < 0) { ny += nb1; sy = -1; } else if (ny >= nb1) { ny -= nb1; sy = 1; }
-            sxa[q] = sx;
-            sya[q] = sy;
-            ns[q] = P.cell_rank[cb + nx + nb0 * (ny + nb1 * nz)];
-        }
-        int st[9], en[9];
+            for (int m = 0; m < NL_ITER; ++m) {
+                const int t = tb + m * WAVE + lane;
+                int k = 0;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            st[q] = P.cell_start[cb + ns[q]];
-            en[q] = P.cell_start[cb + ns[q] + 1];
-        }
-        const float shz = (float)sz * d->L[2];
-        for (int q = 0; q < 9; ++q) {
-            const float shx = (float)sxa[q] * d->L[0], shy = (float)sya[q] * d->L[1];
-            const uint32_t code = (uint32_t)((sxa[q] + 1) + 3 * (sya[q] + 1) + 9 * (sz + 1));
-            // four candidates' positions in flight, tested in order
-            int j = st[q];
-            const int e = en[q];
-            for (; j + 4 <= e; j += 4) {
-                const float4 q0 = pos[j], q1 = pos[j + 1], q2 = pos[j + 2], q3 = pos[j + 3];
-                test(j, q0, shx, shy, shz, code);
-                test(j + 1, q1, shx, shy, shz, code);
-                test(j + 2, q2, shx, shy, shz, code);
-                test(j + 3, q3, shx, shy, shz, code);
+                for (int step = 16; step > 0; step >>= 1) {
+                    const int kk = min(k + step, 26);
+                    if (__shfl(kpre, kk) <= t) k = kk;
+                }
+                // every shuffle with the whole wave active (a lane reads a
+                // source lane's register only while that lane is active)
+                const int st_k = __shfl(kst, k), pre_k = __shfl(kpre, k);
+                const uint32_t code = (uint32_t)__shfl((int)kcode, k);
+                const float sx_k = __shfl(ksx, k), sy_k = __shfl(ksy, k), sz_k = __shfl(ksz, k);
+                const bool valid = t < ntot;
+                const int j = valid ? st_k + (t - pre_k) : c0;
+                const float4 q = pos[j];
+                qx[m] = valid ? q.x + sx_k : __builtin_inff();
+                qy[m] = valid ? q.y + sy_k : __builtin_inff();
+                qz[m] = valid ? q.z + sz_k : __builtin_inff();
+                qe[m] = (int)((uint32_t)j | (code << IMG_SHIFT));
+                qself[m] = code == (uint32_t)IMG_ZERO ? j : -1;
             }
-            for (; j < e; ++j) test(j, pos[j], shx, shy, shz, code);
+            const int nm = min(NL_ITER, (ntot - tb + WAVE - 1) / WAVE);
+            for (int c = cg; c < cg + ng; ++c) {
+                const int cl = c - cg;
+                const float pcx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pg.x), cl));
+                const float pcy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pg.y), cl));
+                const float pcz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pg.z), cl));
+                int cnt = __builtin_amdgcn_readlane(cntv, cl);
+                int *row = P.nlist + nbr_at(c, 0, maxnbr);
+#pragma unroll
+                for (int m = 0; m < NL_ITER; ++m) {
+                    if (m >= nm) break;
+                    const float dx = pcx - qx[m], dy = pcy - qy[m], dz = pcz - qz[m];
+                    const float r2 = (dx * dx + dy * dy) + dz * dz;
+                    const bool acc = r2 <= rc2 && qself[m] != c;
+                    const unsigned long long mask = __builtin_amdgcn_ballot_w64(acc);
+                    if (mask) {
+                        const int p = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                        const uint32_t q = (uint32_t)(cnt + p);
+                        if (acc && q < (uint32_t)maxnbr)
+                            row[(q & ~(uint32_t)(NBR_CHUNK - 1)) * WAVE + (q & (uint32_t)(NBR_CHUNK - 1))] = qe[m];
+                        cnt += __popcll(mask);
+                    }
+                }
+                if (lane == cl) cntv = cnt;
+            }
         }
+        if (lane < ng) P.ncount[cg + lane] = cntv;
+        if (__ballot(lane < ng && cntv > maxnbr) && lane == 0) atomicOr(&d->overflow, 1);
     }
-    P.ncount[i] = cnt;
-    if (cnt > maxnbr) atomicOr(&d->overflow, 1);
 }
 
 // overflow flags of the sorted systems cleared before their rows are rebuilt
@@ -332,22 +421,34 @@ __global__ void __launch_bounds__(256) k_signatures(PoolDev P)
 // host launchers
 // ----------------------------------------------------------------------------
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+static void launch_nlist(const PoolDev &P, int relist, hipStream_t st)
+{
+    if (P.ncact > 0)
+        hipLaunchKernelGGL(k_nlist, dim3(xcd_grid(P.ncact)), dim3(WAVE), 0, st, P, relist, P.ncact);
+}
+
 hipError_t launch_rebuild(const PoolDev &P, hipStream_t st)
 {
     if (P.nact == 0) return hipSuccess;
     const dim3 g(nblk(P.nact, 256)), b(256);
     hipLaunchKernelGGL(k_keys, g, b, 0, st, P);
-    hipLaunchKernelGGL(k_scan, dim3(P.nsys), dim3(1024), 0, st, P);
+    // workgroups per system for the chunked scan: one system spreads over up
+    // to 64, a pool of many systems runs them side by side
+    const int nch = (int)(((long long)P.ncellslots * SUBCELLS + SCAN_CH - 1) / SCAN_CH);
+    const int gs = P.nsys == 1 ? std::max(1, std::min(64, nch)) : 4;
+    hipLaunchKernelGGL(k_scan_sums, dim3(gs, P.nsys), dim3(SCAN_BLK), 0, st, P);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(P.nsys), dim3(SCAN_BLK), 0, st, P);
+    hipLaunchKernelGGL(k_scan_apply, dim3(gs, P.nsys), dim3(SCAN_BLK), 0, st, P);
     hipLaunchKernelGGL(k_scatter, g, b, 0, st, P);
     hipLaunchKernelGGL(k_sortgather, g, b, 0, st, P);
-    hipLaunchKernelGGL(k_nlist, g, b, 0, st, P, 0);
+    launch_nlist(P, 0, st);
     return hipGetLastError();
 }
 
 hipError_t launch_nlist_only(const PoolDev &P, hipStream_t st)
 {
     if (P.nact == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nlist, dim3(nblk(P.nact, 256)), dim3(256), 0, st, P, 0);
+    launch_nlist(P, 0, st);
     return hipGetLastError();
 }
 
@@ -355,7 +456,7 @@ hipError_t launch_relist(const PoolDev &P, hipStream_t st)
 {
     if (P.nact == 0) return hipSuccess;
     hipLaunchKernelGGL(k_clear_overflow, dim3(nblk(P.nsys, 64)), dim3(64), 0, st, P);
-    hipLaunchKernelGGL(k_nlist, dim3(nblk(P.nact, 256)), dim3(256), 0, st, P, 1);
+    launch_nlist(P, 1, st);
     return hipGetLastError();
 }
 

@@ -33,10 +33,10 @@ __global__ void __launch_bounds__(OBS_BLK) k_rdf(PoolDev P, unsigned *hist, int 
             const float4 pi = pos[i];
             const int maxnbr = P.maxnbr;
             const int cnt = min(P.ncount[i], maxnbr);
-            const int *row = P.nlist + (size_t)(i >> 6) * maxnbr * WAVE + (i & 63);
+            const int *row = P.nlist + nbr_at(i, 0, maxnbr);
             const float rc2 = rc * rc, binv = (float)nbin / rc;
             for (int q = 0; q < cnt; ++q) {
-                const uint32_t e = (uint32_t)row[(size_t)q * WAVE];
+                const uint32_t e = (uint32_t)row[(size_t)(q & ~(NBR_CHUNK - 1)) * WAVE + (q & (NBR_CHUNK - 1))];
                 const uint32_t j = e & IDX_MASK;
                 if (!(mask[j] & gj)) continue;
                 const float4 pj = pos[j];

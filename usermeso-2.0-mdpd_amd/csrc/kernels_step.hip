@@ -24,17 +24,6 @@
 namespace mdpd {
 
 constexpr int BLK = 256;
-constexpr int NXCD = 8;
-
-// hardware dispatches workgroup b to XCD b % 8; give XCD x the contiguous
-// logical range [x*per, (x+1)*per)
-__device__ inline int xcd_logical(int b, int nlogical)
-{
-    const int per = (nlogical + NXCD - 1) / NXCD;
-    return (b % NXCD) * per + b / NXCD;
-}
-
-static inline int xcd_grid(int nlogical) { return ((nlogical + NXCD - 1) / NXCD) * NXCD; }
 
 // Per-tile "a particle moved past skin/2" flags: each workgroup ORs its lanes
 // in LDS and stores one word per 64-particle tile it covers (plain stores, no
@@ -73,9 +62,10 @@ __device__ inline float pair_rsq(const float4 &pi, float4 q, uint32_t code, cons
 // entries with r^2 < rc^2 for the force pass.  WITH_RHO = false is the DPD
 // filter pass (no density).
 //
-// One workgroup per 64-particle tile; wave w of SPLIT walks rows p = w mod
-// SPLIT of the tile and writes its compact entries to compact rows
-// t * SPLIT + w, so the force pass can give the same rows to the same wave.
+// One workgroup per 64-particle tile; wave w of SPLIT walks the quads of
+// row entries 4k..4k+3 with k = w mod SPLIT and writes its compact entries
+// to compact rows t * SPLIT + w, so the force pass can give the same rows to
+// the same wave.
 // Partial densities are summed in LDS in wave order (deterministic).
 // --------------------------------------------------------------------------
 template <bool WITH_RHO, int SPLIT>
@@ -96,7 +86,7 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
     const float4 *pos = P.pos[d->sel];
     const int maxnbr = P.maxnbr;
     const size_t tbase = (size_t)tile * maxnbr * WAVE + lane;
-    const int *row = P.nlist + tbase;
+    const int *row = P.nlist + nbr_at(i, 0, maxnbr);
     int *crow = P.clist + tbase;
     int cnt = 0;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -122,26 +112,23 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
             ++nc;
         }
     };
-    // four entries (and their position gathers) in flight per lane, visited
-    // in row order
-    int p = w;
-    for (; p + 3 * SPLIT < cnt; p += 4 * SPLIT) {
-        const uint32_t e0 = (uint32_t)row[(size_t)p * WAVE];
-        const uint32_t e1 = (uint32_t)row[(size_t)(p + SPLIT) * WAVE];
-        const uint32_t e2 = (uint32_t)row[(size_t)(p + 2 * SPLIT) * WAVE];
-        const uint32_t e3 = (uint32_t)row[(size_t)(p + 3 * SPLIT) * WAVE];
+    // a quad of entries (one 16-byte load) and their four position gathers in
+    // flight per lane; wave w takes quads w, w + SPLIT, ... and visits them in
+    // row order.  Entries past the row's end are neither gathered nor visited.
+    for (int p = 4 * w; p < cnt; p += 4 * SPLIT) {
+        const int4 e4 = *(const int4 *)(row + (size_t)(p & ~(NBR_CHUNK - 1)) * WAVE + (p & (NBR_CHUNK - 1)));
+        const uint32_t e0 = (uint32_t)e4.x;
+        const uint32_t e1 = p + 1 < cnt ? (uint32_t)e4.y : (uint32_t)i;
+        const uint32_t e2 = p + 2 < cnt ? (uint32_t)e4.z : (uint32_t)i;
+        const uint32_t e3 = p + 3 < cnt ? (uint32_t)e4.w : (uint32_t)i;
         const float4 q0 = pos[e0 & IDX_MASK];
         const float4 q1 = pos[e1 & IDX_MASK];
         const float4 q2 = pos[e2 & IDX_MASK];
         const float4 q3 = pos[e3 & IDX_MASK];
         visit(e0, q0);
-        visit(e1, q1);
-        visit(e2, q2);
-        visit(e3, q3);
-    }
-    for (; p < cnt; p += SPLIT) {
-        const uint32_t e0 = (uint32_t)row[(size_t)p * WAVE];
-        visit(e0, pos[e0 & IDX_MASK]);
+        if (p + 1 < cnt) visit(e1, q1);
+        if (p + 2 < cnt) visit(e2, q2);
+        if (p + 3 < cnt) visit(e3, q3);
     }
     // coalesced store of the staged rows
     int ncmax = min(nc, STAGE);
@@ -468,11 +455,11 @@ __global__ void __launch_bounds__(WAVE) k_aboundary(PoolDev P, int ntiles, int p
         if (mask[i] & d->fix_mobile_bit) {
             const int maxnbr = P.maxnbr;
             const int cnt = min(P.ncount[i], maxnbr);
-            const int *row = P.nlist + (size_t)tile * maxnbr * WAVE + lane;
+            const int *row = P.nlist + nbr_at(i, 0, maxnbr);
             const float c = d->cut_phi, c2 = c * c;
             float phi = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
             for (int q = 0; q < cnt; ++q) {
-                const uint32_t e = (uint32_t)row[(size_t)q * WAVE];
+                const uint32_t e = (uint32_t)row[(size_t)(q & ~(NBR_CHUNK - 1)) * WAVE + (q & (NBR_CHUNK - 1))];
                 const uint32_t j = e & IDX_MASK;
                 if (!(mask[j] & d->fix_wall_bit)) continue;
                 const float4 pj = pos[j];

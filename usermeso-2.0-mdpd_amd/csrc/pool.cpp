@@ -139,9 +139,12 @@ struct Job {
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
 
-// bytes of device memory per particle slot (state, scratch, 2 cell slots)
-// and per neighbor-row entry (nlist + clist)
-constexpr double BYTES_PER_SLOT = 256.0;
+// bytes of device memory per particle slot (state and scratch), per cell
+// slot (rank, coordinates, system; count and start per sub-cell), and per
+// neighbor-row entry (nlist + clist); CELLS_PER_SLOT cell slots are kept per
+// particle slot
+constexpr double BYTES_PER_SLOT = 240.0;
+constexpr double BYTES_PER_CELL = 12.0 + 8.0 * SUBCELLS;
 constexpr double BYTES_PER_ENTRY = 8.0;
 constexpr int CELLS_PER_SLOT = 2;
 
@@ -158,6 +161,7 @@ struct mdpd_pool {
     bool allocated = false;
     int cap = 0, nslots = 0, maxnbr = 0;
     int nact = 0;                         // launch extent: high-water slot of the resident systems
+    int ncact = 0;                        // per-cell launch extent: high-water cell slot
     DevBuf<float4> pos[2], vel[2], xhold, frc, vend;
     DevBuf<int> tag[2], mask[2], origin[2], image[2];
     DevBuf<float> rho, vir, coef, minv, mmass, phi[2];
@@ -275,7 +279,7 @@ struct mdpd_pool {
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
         P.cl = cl.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
-        P.cap = cap; P.nact = nact; P.ncellslots = nslots; P.nsys = max_sys; P.maxnbr = maxnbr;
+        P.cap = cap; P.nact = nact; P.ncellslots = nslots; P.ncact = ncact; P.nsys = max_sys; P.maxnbr = maxnbr;
         return P;
     }
 
@@ -286,13 +290,13 @@ struct mdpd_pool {
     // particle slots a budget holds at a given row width
     long long capacity_for(int mnb) const
     {
-        const double per = BYTES_PER_SLOT + BYTES_PER_ENTRY * mnb;
+        const double per = BYTES_PER_SLOT + CELLS_PER_SLOT * BYTES_PER_CELL + BYTES_PER_ENTRY * mnb;
         const double c = (budget - fixed_bytes()) / per;
         return c < 256 ? 0 : (long long)(c / 256) * 256;
     }
     double device_bytes() const
     {
-        return fixed_bytes() + (BYTES_PER_SLOT + BYTES_PER_ENTRY * maxnbr) * (double)cap;
+        return fixed_bytes() + (BYTES_PER_SLOT + CELLS_PER_SLOT * BYTES_PER_CELL + BYTES_PER_ENTRY * maxnbr) * (double)cap;
     }
 
     // ------------------------------------------------------------------
@@ -365,12 +369,13 @@ struct mdpd_pool {
         clist.alloc((size_t)cap * maxnbr);
         wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE); status.alloc(max_sys);
         cl.alloc(cap);
-        cell_count.alloc(nslots); cell_start.alloc(nslots); cell_xyz.alloc(nslots);
+        cell_count.alloc((size_t)nslots * SUBCELLS); cell_start.alloc((size_t)nslots * SUBCELLS);
+        cell_xyz.alloc(nslots);
         cell_rank.alloc(nslots); cell_sys.alloc(nslots);
         coef.alloc((size_t)max_sys * MAXTYPES * MAXTYPES * NCOEF);
         minv.alloc((size_t)max_sys * MAXTYPES); mmass.alloc((size_t)max_sys * MAXTYPES);
         dsys.alloc(max_sys);
-        hip_check(hipMemsetAsync(cell_count.p, 0, sizeof(int) * nslots, stream), "memset");
+        hip_check(hipMemsetAsync(cell_count.p, 0, sizeof(int) * (size_t)nslots * SUBCELLS, stream), "memset");
         hip_check(hipMemsetAsync(cell_sys.p, 0xFF, sizeof(int) * nslots, stream), "memset");
         hip_check(hipMemsetAsync(wave_sys.p, 0xFF, sizeof(int) * (cap / WAVE), stream), "memset");
         hip_check(hipMemsetAsync(status.p, 0, sizeof(int) * max_sys, stream), "memset");
@@ -505,7 +510,8 @@ struct mdpd_pool {
         hip_check(hipMemcpyAsync(mmass.p + d.mass_base, ms, sizeof(float) * nt, hipMemcpyHostToDevice, stream), "upload");
         hip_check(hipMemcpyAsync(minv.p + d.mass_base, ms + nt, sizeof(float) * nt, hipMemcpyHostToDevice, stream),
                   "upload");
-        hip_check(hipMemsetAsync(cell_count.p + cbase, 0, sizeof(int) * nc1, stream), "memset");
+        hip_check(hipMemsetAsync(cell_count.p + (size_t)cbase * SUBCELLS, 0, sizeof(int) * (size_t)nc1 * SUBCELLS, stream),
+                  "memset");
         hip_check(hipMemsetAsync(phi[0].p + pbase, 0, sizeof(float) * npad, stream), "memset");
         up(dsys.p + s, 12);
         hipEvent_t e = take_event();
@@ -529,15 +535,19 @@ struct mdpd_pool {
         std::vector<int>().swap(j.ho); std::vector<int>().swap(j.himg);
     }
 
-    // launches cover slots [0, nact): the highest resident range, rounded up
-    // to 64K slots so the captured pass is rebuilt only when it moves by that
+    // launches cover particle slots [0, nact) and cell slots [0, ncact): the
+    // highest resident ranges, rounded up to 64K slots so the captured pass is
+    // rebuilt only when they move by that
     void update_extent()
     {
         const long long hw = parts.high() * 256;
         const long long q = 65536;
         const int want = (int)std::min<long long>(cap, (hw + q - 1) / q * q);
-        if (want != nact) {
+        const long long hc = cells.high();
+        const int wantc = (int)std::min<long long>(nslots, (hc + q - 1) / q * q);
+        if (want != nact || wantc != ncact) {
             nact = want;
+            ncact = wantc;
             graph_valid = false;
         }
     }

@@ -1132,7 +1132,7 @@ struct mdpd_system {
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
         P.cl = cl.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
-        P.cap = cap; P.nact = cap; P.ncellslots = nslots; P.nsys = 1; P.maxnbr = maxnbr;
+        P.cap = cap; P.nact = cap; P.ncellslots = nslots; P.ncact = nslots; P.nsys = 1; P.maxnbr = maxnbr;
         return P;
     }
 
@@ -1321,7 +1321,8 @@ struct mdpd_system {
         clist.alloc((size_t)cap * maxnbr);
         wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE); status.alloc(1);
         cl.alloc(cap);
-        cell_count.alloc(nslots); cell_start.alloc(nslots); cell_xyz.alloc(nslots);
+        cell_count.alloc((size_t)nslots * SUBCELLS); cell_start.alloc((size_t)nslots * SUBCELLS);
+        cell_xyz.alloc(nslots);
         cell_rank.alloc(nslots); cell_sys.alloc(nslots);
         coef.alloc(ds.coef.size()); minv.alloc(ntypes); mmass.alloc(ntypes); dsys.alloc(1);
 
@@ -1346,7 +1347,7 @@ struct mdpd_system {
         up(coef.p, ds.coef.data(), ds.coef.size() * sizeof(float));
         up(minv.p, ds.minv.data(), ntypes * sizeof(float));
         up(mmass.p, ds.mass.data(), ntypes * sizeof(float));
-        hip_check(hipMemsetAsync(cell_count.p, 0, nslots * sizeof(int), stream), "memset");
+        hip_check(hipMemsetAsync(cell_count.p, 0, (size_t)nslots * SUBCELLS * sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(ncount.p, 0, cap * sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(rho.p, 0, cap * sizeof(float), stream), "memset");
         hip_check(hipMemsetAsync(phi[0].p, 0, cap * sizeof(float), stream), "memset");
