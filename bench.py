@@ -45,6 +45,11 @@ NLIST = RHO * 4.0 / 3.0 * np.pi * (RC + SKIN) ** 3          # 55.7 entries per r
 B_FORCE = 12 + 12 + 4 + 4 + 4 + 12 + 12 + 4 * NLIST          # bytes per particle, force + fused VV pass
 B_DENSITY = 12 + 4 + 4 * NLIST                               # bytes per particle, density pass
 B_STEP = B_FORCE + B_DENSITY                                 # 522 B per particle-step
+# HBM bytes per particle moved by the pooled force pass, measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (medians over the passes of 64
+# droplets, counters scaled by a 1 GiB copy: x2.0 fetch, x1.0 write):
+# 192.6 read + 32.1 written
+T_FORCE_POOL = 192.6 + 32.1
 GB = float(1 << 30)
 
 SLABS = {
@@ -196,11 +201,13 @@ def plain_sample(sizes, spec, steps, k):
 # ---------------------------------------------------------------------------
 # batched: one pool, all jobs resident
 # ---------------------------------------------------------------------------
-def batched(specs, steps, warmup, split, budget):
+def batched(specs, steps, warmup, split, budget, graph=True):
     import mdpd355
     # warm-up: a small pool of the first jobs (loads every kernel, builds a graph)
     if warmup > 0:
         w = mdpd355.Pool(budget, 16)
+        if not graph:
+            w.tune("graph", 0)
         for box, x, v, seed in specs[:8]:
             s = system(box, x, v, seed)
             w.submit(s, warmup)
@@ -210,6 +217,8 @@ def batched(specs, steps, warmup, split, budget):
     p = mdpd355.Pool(budget, max(len(specs), 1))
     p.tune("split", split or 0)
     p.tune("profile", 16)
+    if not graph:
+        p.tune("graph", 0)
     ids = []
     for box, x, v, seed in specs:
         s = system(box, x, v, seed)
@@ -312,10 +321,11 @@ def cpu_baseline(box, x, v, lucy, label, target_s=12.0, seed=4242):
                       f"after setup, {t:.1f} s wall (setup {t_setup:.1f} s excluded)"}
 
 
-def roofline(t_force_s, particles, kernel):
+def roofline(t_force_s, particles, kernel, traffic_per_particle=None):
     achieved = B_FORCE * particles / t_force_s / 1e9
+    traffic = None if traffic_per_particle is None else traffic_per_particle * particles
     return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_particle": B_FORCE,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_particle": B_FORCE,
             "particles_per_launch": particles, "avg_launch_us": t_force_s * 1e6}
 
 
@@ -332,7 +342,7 @@ def run_batched(a):
     sizes = batch_sizes(a.jobs, a.seed)
     specs = [(*make_droplet(int(n), a.seed + 1 + k), 5000 + k) for k, n in enumerate(sizes)]
     total = int(sizes.sum())
-    dt, st, temps = batched(specs, a.steps, a.warmup, a.split, a.budget_gb * GB)
+    dt, st, temps = batched(specs, a.steps, a.warmup, a.split, a.budget_gb * GB, graph=not a.no_graph)
     value = total * a.steps / dt
     plain_value, plain_ns, plain_s = (plain_sample(sizes, lambda i: specs[i], a.steps, a.plain_sample)
                                       if a.plain_sample > 0 else (None, [], 0.0))
@@ -363,7 +373,7 @@ def run_batched(a):
         "final_temp_sample": temps,
         "served": srv,
     }
-    roof = roofline(t_force, total, "k_force<mDPD, MODE_PHASED> (pool pass, all systems)")
+    roof = roofline(t_force, total, "k_force<mDPD, MODE_PHASED> (pool pass, all systems)", T_FORCE_POOL)
     cpu = None
     if not a.no_cpu:
         med = int(np.argsort(sizes)[len(sizes) // 2])
@@ -535,6 +545,8 @@ def main():
     ap.add_argument("--served-rate", type=float, default=50.0)
     ap.add_argument("--served-budget-gb", type=float, default=4.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched leg: launch each pass's kernels directly instead of replaying a hipGraph")
     ap.add_argument("--split", type=int, default=None, help="waves per tile (default: autotuned / 2)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device with a plain copy kernel (a known-size "
