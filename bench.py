@@ -364,7 +364,7 @@ def run_batched(a):
                         f"{a.steps} steps, {plain_s:.1f} s",
         "passes": st["passes"], "split": st["split"], "pool_capacity": st["capacity"], "maxnbr": st["maxnbr"],
         "pool_GB": st["bytes"] / GB, "regrows": st["regrows"],
-        "tune_ms_per_pass": {"1": st["tune_ms_1"], "2": st["tune_ms_2"], "4": st["tune_ms_4"]},
+        "tune_first_force_ms_per_pass": {"1": st["tune_ms_1"], "2": st["tune_ms_2"], "4": st["tune_ms_4"]},
         "sampled_ms_per_pass": {k: st[f"prof_{k}_ms"] / max(st["prof_passes"], 1)
                                 for k in ("rebuild", "first", "force", "commit")},
         "sampled_passes": st["prof_passes"],

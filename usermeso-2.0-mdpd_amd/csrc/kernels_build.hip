@@ -124,9 +124,14 @@ __global__ void __launch_bounds__(SCAN_BLK) k_scan_sums(PoolDev P)
     __shared__ int wsum[SCAN_BLK / 64];
     const int nc = d->ncell * SUBCELLS, base = d->cell_base * SUBCELLS;
     for (int c0 = blockIdx.x * SCAN_CH; c0 < nc; c0 += gridDim.x * SCAN_CH) {
+        // clamped loads, all in flight; entries past the end count 0
         int sum = 0;
-        for (int k = threadIdx.x; k < SCAN_CH; k += SCAN_BLK)
-            if (c0 + k < nc) sum += P.cell_count[base + c0 + k];
+#pragma unroll
+        for (int u = 0; u < SCAN_PER; ++u) {
+            const int c = c0 + u * SCAN_BLK + (int)threadIdx.x;
+            const int v = P.cell_count[base + min(c, nc - 1)];
+            sum += c < nc ? v : 0;
+        }
         int total;
         block_excl_scan(sum, wsum, &total);
         if (threadIdx.x == 0) P.cell_start[base + c0] = total;
@@ -164,10 +169,16 @@ __global__ void __launch_bounds__(SCAN_BLK) k_scan_apply(PoolDev P)
         const int off = P.cell_start[base + c0];     // the chunk's offset
         // coalesced load into LDS (counts cleared), SCAN_PER consecutive
         // counts per thread, workgroup scan, coalesced store
-        for (int k = t; k < SCAN_CH; k += SCAN_BLK) {
-            int v = 0;
-            if (c0 + k < nc) { v = P.cell_count[base + c0 + k]; P.cell_count[base + c0 + k] = 0; }
-            buf[k] = v;
+#pragma unroll
+        for (int u = 0; u < SCAN_PER; ++u) {
+            const int c = c0 + u * SCAN_BLK + t;
+            const int v = P.cell_count[base + min(c, nc - 1)];
+            buf[u * SCAN_BLK + t] = c < nc ? v : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < SCAN_PER; ++u) {
+            const int c = c0 + u * SCAN_BLK + t;
+            if (c < nc) P.cell_count[base + c] = 0;
         }
         __syncthreads();
         int v[SCAN_PER], sum = 0;
@@ -176,8 +187,11 @@ __global__ void __launch_bounds__(SCAN_BLK) k_scan_apply(PoolDev P)
         int run = off + block_excl_scan(sum, wsum, &total);
         for (int k = 0; k < SCAN_PER; ++k) { buf[t * SCAN_PER + k] = run; run += v[k]; }
         __syncthreads();
-        for (int k = t; k < SCAN_CH; k += SCAN_BLK)
+#pragma unroll
+        for (int u = 0; u < SCAN_PER; ++u) {
+            const int k = u * SCAN_BLK + t;
             if (c0 + k < nc) P.cell_start[base + c0 + k] = buf[k];
+        }
         __syncthreads();
     }
 }
@@ -261,14 +275,17 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 // candidates 64 at a time, one per lane (coalesced loads, image shift applied
 // once), NL_ITER iterations held in registers, and tests them against one
 // centre after another (the centre's position is uniform).  The lanes that
-// pass append in lane order (ballot + mbcnt), which keeps the row order.
+// pass append in lane order (ballot + mbcnt) to an LDS copy of the centre's
+// row, which keeps the row order; the copy is stored to the chunked row
+// layout once per centre and batch.  The workgroup is one wave.
 // relist: rows of the current, already sorted buffer (the row width grew);
 // otherwise of the data k_sortgather just wrote.
 // --------------------------------------------------------------------------
 constexpr int NL_ITER = 8;
 
-__global__ void __launch_bounds__(WAVE) k_nlist(PoolDev P, int relist, int ncells)
+__global__ void __launch_bounds__(WAVE, 6) k_nlist(PoolDev P, int relist, int ncells)
 {
+    __shared__ int s_row[NL_ITER * WAVE];                   // one centre's entries of one batch
     const int cs = xcd_logical(blockIdx.x, ncells);        // cell slot (Morton rank order)
     if (cs >= ncells) return;
     const int s = P.cell_sys[cs];
@@ -306,6 +323,9 @@ __global__ void __launch_bounds__(WAVE) k_nlist(PoolDev P, int relist, int ncell
     const int kend = wave_incl_scan(klen, lane);
     const int kpre = kend - klen;                           // candidates before cell k
     const int ntot = __builtin_amdgcn_readlane(kend, 26);
+    // the centre's own cell is stencil cell 13 (no image shift): centre c is
+    // candidate pre13 + (c - c0) of its own list
+    const int pre13 = __builtin_amdgcn_readlane(kpre, 13);
     const float rc2 = d->rc2_list;
     const int maxnbr = P.maxnbr;
     for (int cg = c0; cg < c1; cg += WAVE) {
@@ -316,7 +336,7 @@ __global__ void __launch_bounds__(WAVE) k_nlist(PoolDev P, int relist, int ncell
             // candidates tb + 64 m + lane: stencil cell by binary search over
             // the exclusive prefix (the last cell starting at or before t)
             float qx[NL_ITER], qy[NL_ITER], qz[NL_ITER];
-            int qe[NL_ITER], qself[NL_ITER];
+            int qe[NL_ITER];
 #pragma unroll
             for (int m = 0; m < NL_ITER; ++m) {
                 const int t = tb + m * WAVE + lane;
@@ -338,7 +358,6 @@ __global__ void __launch_bounds__(WAVE) k_nlist(PoolDev P, int relist, int ncell
                 qy[m] = valid ? q.y + sy_k : __builtin_inff();
                 qz[m] = valid ? q.z + sz_k : __builtin_inff();
                 qe[m] = (int)((uint32_t)j | (code << IMG_SHIFT));
-                qself[m] = code == (uint32_t)IMG_ZERO ? j : -1;
             }
             const int nm = min(NL_ITER, (ntot - tb + WAVE - 1) / WAVE);
             for (int c = cg; c < cg + ng; ++c) {
@@ -346,25 +365,36 @@ __global__ void __launch_bounds__(WAVE) k_nlist(PoolDev P, int relist, int ncell
                 const float pcx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pg.x), cl));
                 const float pcy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pg.y), cl));
                 const float pcz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pg.z), cl));
-                int cnt = __builtin_amdgcn_readlane(cntv, cl);
-                int *row = P.nlist + nbr_at(c, 0, maxnbr);
+                const int q0 = __builtin_amdgcn_readlane(cntv, cl);     // row length before this batch
+                const int tself = pre13 + (c - c0);
+                int n = 0;                                  // entries of this batch, staged in LDS
 #pragma unroll
                 for (int m = 0; m < NL_ITER; ++m) {
                     if (m >= nm) break;
                     const float dx = pcx - qx[m], dy = pcy - qy[m], dz = pcz - qz[m];
                     const float r2 = (dx * dx + dy * dy) + dz * dz;
-                    const bool acc = r2 <= rc2 && qself[m] != c;
-                    const unsigned long long mask = __builtin_amdgcn_ballot_w64(acc);
+                    uint64_t mask = __builtin_amdgcn_ballot_w64(r2 <= rc2);
+                    const int sl = tself - (tb + m * WAVE);  // the centre itself: never listed
+                    if ((unsigned)sl < (unsigned)WAVE) mask &= ~(1ull << sl);
                     if (mask) {
-                        const int p = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                        const uint32_t q = (uint32_t)(cnt + p);
-                        if (acc && q < (uint32_t)maxnbr)
-                            row[(q & ~(uint32_t)(NBR_CHUNK - 1)) * WAVE + (q & (uint32_t)(NBR_CHUNK - 1))] = qe[m];
-                        cnt += __popcll(mask);
+                        if (__builtin_amdgcn_inverse_ballot_w64(mask))
+                            s_row[n + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u))] = qe[m];
+                        n += __popcll(mask);
                     }
                 }
-                if (lane == cl) cntv = cnt;
+                // this batch's entries of centre c: rows [q0, q0 + n) in the
+                // chunked layout, entries past the row width dropped (the
+                // count marks the overflow)
+                __syncthreads();
+                int *row = P.nlist + nbr_at(c, 0, maxnbr);
+                for (int k = lane; k < n; k += WAVE) {
+                    const uint32_t q = (uint32_t)(q0 + k);
+                    if (q < (uint32_t)maxnbr)
+                        row[(q & ~(uint32_t)(NBR_CHUNK - 1)) * WAVE + (q & (uint32_t)(NBR_CHUNK - 1))] = s_row[k];
+                }
+                __syncthreads();
+                if (lane == cl) cntv = q0 + n;
             }
         }
         if (lane < ng) P.ncount[cg + lane] = cntv;

@@ -669,30 +669,35 @@ struct mdpd_pool {
 
     // time each split on a few real passes (the work is not repeated: every
     // pass advances the pool); results differ between splits only by FP32
-    // summation order
+    // summation order.  Only the split-dependent part of a pass is compared
+    // (first pass + force pass): the rebuild kernels do not depend on the
+    // split, and the share of systems re-sorting varies from pass to pass.
     void tune()
     {
         tuned = true;
         if (!autotune) return;
         const int cand[3] = {1, 2, 4};
         double ms[3] = {0, 0, 0};
-        hipEvent_t a, b;
-        hip_check(hipEventCreate(&a), "hipEventCreate");
-        hip_check(hipEventCreate(&b), "hipEventCreate");
+        std::vector<PassEvents> evs;
         for (int rep = 0; rep < 2; ++rep)
             for (int k = 0; k < 3; ++k) {
                 const int c = rep ? cand[2 - k] : cand[k];
-                hip_check(hipEventRecord(a, stream), "hipEventRecord");
-                for (int p = 0; p < 4; ++p) launch_pass(c);
-                hip_check(hipEventRecord(b, stream), "hipEventRecord");
-                hip_check(hipEventSynchronize(b), "tune");
-                float t = 0;
-                hip_check(hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
-                ms[rep ? 2 - k : k] += t;
+                evs.clear();
+                for (int p = 0; p < 4; ++p) {
+                    PassEvents pe;
+                    for (auto &e : pe.e) e = timing_event();
+                    launch_pass(c, &pe);
+                    evs.push_back(pe);
+                }
+                hip_check(hipEventSynchronize(evs.back().e[PP_COUNT]), "tune");
+                for (auto &pe : evs) {
+                    float t = 0;
+                    hip_check(hipEventElapsedTime(&t, pe.e[PP_FIRST], pe.e[PP_COMMIT]), "hipEventElapsedTime");
+                    ms[rep ? 2 - k : k] += t;
+                    for (auto e : pe.e) prof_free.push_back(e);
+                }
                 passes += 4;
             }
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
         int best = 0;
         for (int k = 1; k < 3; ++k)
             if (ms[k] < ms[best]) best = k;
