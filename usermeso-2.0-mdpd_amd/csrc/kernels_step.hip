@@ -62,8 +62,8 @@ __device__ inline float pair_rsq(const float4 &pi, float4 q, uint32_t code, cons
 // entries with r^2 < rc^2 for the force pass.  WITH_RHO = false is the DPD
 // filter pass (no density).
 //
-// One workgroup per 64-particle tile; wave w of SPLIT walks the quads of
-// row entries 4k..4k+3 with k = w mod SPLIT and writes its compact entries
+// One workgroup per 64-particle tile; wave w of SPLIT walks the chunks of
+// row entries 8k..8k+7 with k = w mod SPLIT and writes its compact entries
 // to compact rows t * SPLIT + w, so the force pass can give the same rows to
 // the same wave.
 // Partial densities are summed in LDS in wave order (deterministic).
@@ -112,23 +112,31 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
             ++nc;
         }
     };
-    // a quad of entries (one 16-byte load) and their four position gathers in
-    // flight per lane; wave w takes quads w, w + SPLIT, ... and visits them in
-    // row order.  Entries past the row's end are neither gathered nor visited.
-    for (int p = 4 * w; p < cnt; p += 4 * SPLIT) {
-        const int4 e4 = *(const int4 *)(row + (size_t)(p & ~(NBR_CHUNK - 1)) * WAVE + (p & (NBR_CHUNK - 1)));
-        const uint32_t e0 = (uint32_t)e4.x;
-        const uint32_t e1 = p + 1 < cnt ? (uint32_t)e4.y : (uint32_t)i;
-        const uint32_t e2 = p + 2 < cnt ? (uint32_t)e4.z : (uint32_t)i;
-        const uint32_t e3 = p + 3 < cnt ? (uint32_t)e4.w : (uint32_t)i;
-        const float4 q0 = pos[e0 & IDX_MASK];
-        const float4 q1 = pos[e1 & IDX_MASK];
-        const float4 q2 = pos[e2 & IDX_MASK];
-        const float4 q3 = pos[e3 & IDX_MASK];
-        visit(e0, q0);
-        if (p + 1 < cnt) visit(e1, q1);
-        if (p + 2 < cnt) visit(e2, q2);
-        if (p + 3 < cnt) visit(e3, q3);
+    // a chunk of eight entries (two 16-byte loads) and their eight position
+    // gathers in flight per lane, the next chunk's entries loading meanwhile;
+    // wave w takes chunks w, w + SPLIT, ... and visits them in row order.
+    // Entries past the row's end are neither gathered nor visited.
+    const int nch = (cnt + NBR_CHUNK - 1) / NBR_CHUNK;
+    const int4 *row4 = (const int4 *)row;
+    int4 ea = make_int4(0, 0, 0, 0), eb = ea;
+    if (w < nch) {
+        ea = row4[(size_t)w * (NBR_CHUNK * WAVE / 4)];
+        eb = row4[(size_t)w * (NBR_CHUNK * WAVE / 4) + 1];
+    }
+    for (int c = w; c < nch; c += SPLIT) {
+        const int p = c * NBR_CHUNK;
+        const uint32_t e[NBR_CHUNK] = {(uint32_t)ea.x, (uint32_t)ea.y, (uint32_t)ea.z, (uint32_t)ea.w,
+                                       (uint32_t)eb.x, (uint32_t)eb.y, (uint32_t)eb.z, (uint32_t)eb.w};
+        if (c + SPLIT < nch) {
+            ea = row4[(size_t)(c + SPLIT) * (NBR_CHUNK * WAVE / 4)];
+            eb = row4[(size_t)(c + SPLIT) * (NBR_CHUNK * WAVE / 4) + 1];
+        }
+        float4 q[NBR_CHUNK];
+#pragma unroll
+        for (int u = 0; u < NBR_CHUNK; ++u) q[u] = pos[(p + u < cnt ? e[u] : (uint32_t)i) & IDX_MASK];
+#pragma unroll
+        for (int u = 0; u < NBR_CHUNK; ++u)
+            if (p + u < cnt) visit(e[u], q[u]);
     }
     // coalesced store of the staged rows
     int ncmax = min(nc, STAGE);
