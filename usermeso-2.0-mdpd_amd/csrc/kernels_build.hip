@@ -337,8 +337,10 @@ __global__ void __launch_bounds__(WAVE, 6) k_nlist(PoolDev P, int relist, int nc
             // the exclusive prefix (the last cell starting at or before t)
             float qx[NL_ITER], qy[NL_ITER], qz[NL_ITER];
             int qe[NL_ITER];
+            const int nm = min(NL_ITER, (ntot - tb + WAVE - 1) / WAVE);
 #pragma unroll
             for (int m = 0; m < NL_ITER; ++m) {
+                if (m >= nm) break;
                 const int t = tb + m * WAVE + lane;
                 int k = 0;
 #pragma unroll
@@ -359,7 +361,6 @@ __global__ void __launch_bounds__(WAVE, 6) k_nlist(PoolDev P, int relist, int nc
                 qz[m] = valid ? q.z + sz_k : __builtin_inff();
                 qe[m] = (int)((uint32_t)j | (code << IMG_SHIFT));
             }
-            const int nm = min(NL_ITER, (ntot - tb + WAVE - 1) / WAVE);
             for (int c = cg; c < cg + ng; ++c) {
                 const int cl = c - cg;
                 const float pcx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pg.x), cl));

@@ -147,6 +147,7 @@ constexpr double BYTES_PER_SLOT = 240.0;
 constexpr double BYTES_PER_CELL = 12.0 + 8.0 * SUBCELLS;
 constexpr double BYTES_PER_ENTRY = 8.0;
 constexpr int CELLS_PER_SLOT = 2;
+constexpr long long RETUNE_PASSES = 256;
 
 }  // namespace
 
@@ -179,7 +180,8 @@ struct mdpd_pool {
 
     // ---- launch shape ----
     int split = 2;
-    bool autotune = true, tuned = false;
+    bool autotune = true, tuned = false, retuned = false;
+    long long retune_at = 0;              // passes after which the split is tuned once more
     double tune_ms[5] = {0, 0, 0, 0, 0};
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
@@ -892,8 +894,13 @@ struct mdpd_pool {
                 fail(MDPD_E_NOMEM, "job %d does not fit the free pool", queue.front());
             }
             if (max_passes > 0 && launched >= max_passes) break;
-            if (!tuned) {
+            // the first passes after a start from raw input are not typical
+            // (rows relax, most systems re-sort every pass): the split is
+            // tuned again once the pool has run RETUNE_PASSES passes
+            if (!tuned || (autotune && !retuned && passes >= retune_at)) {
+                if (tuned) retuned = true;
                 tune();
+                retune_at = passes + RETUNE_PASSES;
                 launched += 24;
                 poll();
                 continue;
@@ -1088,6 +1095,7 @@ int mdpd_pool_tune(mdpd_pool *pool, const char *knob, int value)
             if (value != 0 && value != 1 && value != 2 && value != 4) fail(MDPD_E_ARG, "split must be 0 (auto), 1, 2 or 4");
             pool->autotune = value == 0;
             pool->tuned = value != 0;
+            pool->retuned = false;
             if (value) pool->split = value;
             pool->graph_valid = false;
         } else if (k == "graph") {
