@@ -312,7 +312,7 @@ def cpu_baseline(box, x, v, lucy, label, target_s=12.0, seed=4242):
     t0 = time.perf_counter()
     orc.run(x, v, *args, steps, **kw)
     per = max((time.perf_counter() - t0 - t_setup) / steps, 1e-6)
-    steps = int(max(2, min(2000, target_s / per)))
+    steps = int(max(2, min(6000, target_s / per)))
     t0 = time.perf_counter()
     orc.run(x, v, *args, steps, **kw)
     t = time.perf_counter() - t0 - t_setup
