@@ -252,7 +252,6 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0):
     sizes = np.round(np.exp(rng.uniform(np.log(1000.0), np.log(200000.0), len(arr)))).astype(int)
     spec = lambda i: (make_droplet(sizes[i], 7000 + i) if kinds[i] else make_slab_job(sizes[i], 7000 + i))
     p = mdpd355.Pool(budget, 1024)
-    p.tune("split", 2)
     p.tune("poll", 4)
     meta, lat, work = {}, [], 0.0
     t0 = mdpd355.clock()

@@ -261,7 +261,9 @@ __device__ inline float ln_u32(uint32_t w)
     const uint32_t pk = k >= 32 ? 0u : (1u << k);
     const float f = __builtin_amdgcn_ldexpf((float)(int)(w - pk), -k);
     const float w1 = 1.0f + f;
-    const float lp = __logf(w1) * f * __builtin_amdgcn_rcpf(w1 - 1.0f);
+    // w1 lies in [1/sqrt2, sqrt2]: the hardware log2 needs no range or
+    // denormal handling there
+    const float lp = __builtin_amdgcn_logf(w1) * 0.69314718055994530942f * f * __builtin_amdgcn_rcpf(w1 - 1.0f);
     return (float)(k - 32) * 0.69314718055994530942f + (w1 == 1.0f ? f : lp);
 }
 

@@ -29,8 +29,11 @@ __global__ void __launch_bounds__(256) k_keys(PoolDev P)
     if (s < 0) return;
     SysDev *d = P.sys + s;
     if (!d->active || !d->stale) return;
-    if (i >= d->part_base + d->n) return;
-
+    // the wave stays whole (the slot's run below is found across lanes);
+    // lanes past the system's last particle only take part in that
+    const bool valid = i < d->part_base + d->n;
+    int gc = -1;
+    if (valid) {
     const int cur = d->sel, cura = d->sela;
     float4 p = P.pos[cur][i];
     int img = P.image[cura][i];
@@ -71,10 +74,28 @@ __global__ void __launch_bounds__(256) k_keys(PoolDev P)
         P.key[i] = (z1 << d->l1_shift) | z2;
     }
     const int lin = (b[0] - 1) + d->nb[0] * ((b[1] - 1) + d->nb[1] * (b[2] - 1));
-    const int gc = (d->cell_base + P.cell_rank[d->cell_base + lin]) * SUBCELLS + (int)(z2 >> SUBCELL_SHIFT);
+    gc = (d->cell_base + P.cell_rank[d->cell_base + lin]) * SUBCELLS + (int)(z2 >> SUBCELL_SHIFT);
     P.pcell[i] = gc;
     P.psub[i] = (int)z2;
-    P.pslot[i] = atomicAdd(P.cell_count + gc, 1);
+    }
+    // arrival slots: the particles are still nearly in the last build's
+    // order, so neighboring lanes mostly share a sub-cell; each run of equal
+    // sub-cells takes its slots with one atomic (the order of arrival inside
+    // a sub-cell does not matter: the rank in k_sortgather decides)
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(gc, 1, 64);
+    const bool head = valid && (lane == 0 || prev != gc);
+    const uint64_t heads = __builtin_amdgcn_ballot_w64(head);
+    const uint64_t below = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);    // lanes 0..lane
+    const uint64_t hb = heads & below;
+    const int h = hb ? 63 - __clzll(hb) : 0;                                // this lane's run head
+    const uint64_t later = heads & ~below;
+    const int nvalid = __popcll(__builtin_amdgcn_ballot_w64(valid));
+    const int next = later ? min(__ffsll((long long)later) - 1, nvalid) : nvalid;
+    int base = 0;
+    if (head) base = atomicAdd(P.cell_count + gc, next - lane);
+    base = __shfl(base, h, 64);
+    if (valid) P.pslot[i] = base + (lane - h);
 }
 
 // --------------------------------------------------------------------------
