@@ -256,10 +256,20 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
     const int key = P.cl[f];
     const int cell = P.pcell[idx];
     const int start = P.cell_start[cell], end = P.cell_start[cell + 1];
+    // four members' keys in flight per round trip (loads clamped to the
+    // segment, members past its end not counted)
     int rank = 0;
-    for (int g = start; g < end; ++g) {
-        const int kg = P.cl[g], jg = P.tmp[g];
-        rank += (kg < key) || (kg == key && jg < idx);
+    for (int g0 = start; g0 < end; g0 += 4) {
+        int kg[4], jg[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int g = min(g0 + u, end - 1);
+            kg[u] = P.cl[g];
+            jg[u] = P.tmp[g];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            rank += g0 + u < end && ((kg[u] < key) || (kg[u] == key && jg[u] < idx));
     }
     const int dst = start + rank;
     const int cur = d->sel, cura = d->sela;
