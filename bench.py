@@ -254,6 +254,7 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0):
     p = mdpd355.Pool(budget, 1024)
     p.tune("poll", 4)
     meta, lat, work = {}, [], 0.0
+    occ = []                                   # (used slot fraction, running jobs, queued jobs) per loop
     t0 = mdpd355.clock()
     i = 0
     while True:
@@ -266,6 +267,7 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0):
             i += 1
         if p.stat("running") + p.stat("queued") > 0:
             p.advance(max_passes=8)
+            occ.append((p.stat("used_slots") / max(p.stat("capacity"), 1.0), p.stat("running"), p.stat("queued")))
         elif i < len(arr):
             time.sleep(min(0.002, max(0.0, arr[i] - now)))
         for j in p.done():
@@ -288,6 +290,9 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0):
         "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
         "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
         "offered_particle_steps_per_s": float(sizes.sum()) * steps / seconds if len(arr) else 0.0,
+        "mean_used_slot_fraction": float(np.mean([o[0] for o in occ])) if occ else None,
+        "mean_running_jobs": float(np.mean([o[1] for o in occ])) if occ else None,
+        "mean_queued_jobs": float(np.mean([o[2] for o in occ])) if occ else None,
     }
     return out, sizes, (lambda i: (*spec(i), 9000 + i))
 

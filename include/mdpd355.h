@@ -180,7 +180,9 @@ int mdpd_tea_premix(int device, int n, const unsigned int *a, const unsigned int
  *
  * A job is submitted from a configured handle (script state + atoms are
  * copied; the handle stays the caller's and is not changed), is admitted when
- * its particles and cells fit the pool's HBM budget (first in, first out),
+ * its particles and cells fit the pool's HBM budget (in arrival order; a job
+ * that does not fit yet lets later jobs that do fit pass it, at most 64
+ * times before it goes first),
  * keeps its state resident while it runs, and leaves when done; its results
  * stay readable until mdpd_pool_release.  One host thread drives a pool. */
 typedef struct mdpd_pool mdpd_pool;

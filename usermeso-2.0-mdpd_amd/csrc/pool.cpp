@@ -148,6 +148,8 @@ constexpr double BYTES_PER_CELL = 12.0 + 8.0 * SUBCELLS;
 constexpr double BYTES_PER_ENTRY = 8.0;
 constexpr int CELLS_PER_SLOT = 2;
 constexpr long long RETUNE_PASSES = 256;
+constexpr size_t ADMIT_LOOKAHEAD = 64;    // queued jobs considered per admission round
+constexpr int MAX_BYPASS = 64;            // admissions past one waiting head job
 
 }  // namespace
 
@@ -175,6 +177,7 @@ struct mdpd_pool {
     // ---- jobs ----
     std::map<int, std::unique_ptr<Job>> jobs;
     std::deque<int> queue;
+    int head_bypass = 0;                  // jobs admitted past the current head of the queue
     std::vector<int> finished;            // done since the last mdpd_pool_done call
     int next_id = 1, nrunning = 0, nharvest = 0;
 
@@ -424,24 +427,36 @@ struct mdpd_pool {
     // ------------------------------------------------------------------
     // admission: FIFO, a job waits until its particle and cell ranges fit
     // ------------------------------------------------------------------
+    // Admission in arrival order with bounded backfill: when the oldest
+    // queued job does not fit (its particle or cell range is not free in one
+    // piece), later jobs among the next ADMIT_LOOKAHEAD that do fit are
+    // admitted past it, at most MAX_BYPASS times for the same head job; after
+    // that only the head is admitted until it fits.
     bool admit()
     {
         bool any = false;
         if (!queue.empty() && !allocated) allocate();
-        while (!queue.empty() && !free_slots.empty()) {
-            Job &j = *jobs[queue.front()];
+        size_t k = 0;
+        while (k < queue.size() && k < ADMIT_LOOKAHEAD && !free_slots.empty()) {
+            if (k > 0 && head_bypass >= MAX_BYPASS) break;
+            Job &j = *jobs[queue[k]];
             const long long pu = j.npad / 256;
             const long long pb = parts.take(pu);
-            if (pb < 0) break;
-            const long long cb = cells.take(j.ncell + 1);
-            if (cb < 0) {
-                parts.give(pb, pu);
-                break;
+            long long cb = -1;
+            if (pb >= 0) {
+                cb = cells.take(j.ncell + 1);
+                if (cb < 0) parts.give(pb, pu);
+            }
+            if (pb < 0 || cb < 0) {
+                ++k;
+                continue;
             }
             if (j.ds.want_maxnbr > maxnbr) regrow(j.ds.want_maxnbr);
             const int s = free_slots.back();
             free_slots.pop_back();
-            queue.pop_front();
+            if (k == 0) head_bypass = 0;
+            else ++head_bypass;
+            queue.erase(queue.begin() + (long)k);
             upload(j, s, pb * 256, cb);
             any = true;
         }
