@@ -149,3 +149,29 @@ def test_served_submission_while_running(engine):
     info = [p.info(j) for j in ids]
     assert all(i["t_submit"] <= i["t_admit"] <= i["t_done"] for i in info)
     assert sorted(p.done()) == sorted(ids)
+
+
+def test_backfill_admits_past_a_job_that_does_not_fit(engine):
+    # job b does not fit beside the running job a; the later, smaller job c
+    # does and is admitted past b; b runs when a's range is free.  Every
+    # result equals the job run alone.
+    p = engine.Pool(9.0e6, 4)
+    p.tune("split", 2)
+    p.tune("poll", 4)
+    pad = lambda n: (n + 255) // 256 * 256
+    na, nc = 3000, 200
+    ja = p.submit(droplet(engine, na, 40), 20)
+    cap = int(p.stat("capacity"))
+    nb = cap - pad(na) + 256                      # fits alone, not beside a
+    assert pad(na) + pad(nb) > cap >= pad(nb) and pad(na) + pad(nc) <= cap
+    jb = p.submit(droplet(engine, nb, 41), 10)
+    jc = p.submit(droplet(engine, nc, 42), 15)
+    p.admit()
+    assert (p.state(ja), p.state(jb), p.state(jc)) == (engine.JOB_RUNNING, engine.JOB_QUEUED, engine.JOB_RUNNING)
+    p.run()
+    for j, (n, seed, k) in zip((ja, jb, jc), ((na, 40, 20), (nb, 41, 10), (nc, 42, 15))):
+        assert p.state(j) == engine.JOB_DONE
+        got = ({q: p.gather(j, q) for q in ("x", "v", "f", "image", "e_pair", "virial")}, p.thermo(j),
+               p.info(j)["timestep"])
+        alone = pool_results(engine, [droplet(engine, n, seed)], [k])[0][0]
+        assert_same(got, alone, f"job of {n} atoms")
