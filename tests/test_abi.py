@@ -151,8 +151,10 @@ def test_pool_queue_and_errors_without_a_device(engine):
     j = p.submit(_job(engine), 10)
     assert p.state(j) == engine.JOB_QUEUED
     assert p.stat("queued") == 1 and p.stat("running") == 0
-    # 64 MB at 256 + 8*112 bytes per slot
-    assert 50000 < p.stat("capacity") < 64e6 / 1000
+    # 64 MB at ~330 + 8*maxnbr bytes per slot; rows are a multiple of 32 wide
+    # (a split first pass keeps each wave's compact rows inside the tile)
+    assert p.stat("maxnbr") > 0 and p.stat("maxnbr") % 32 == 0
+    assert 40000 < p.stat("capacity") < 64e6 / 1000
     with pytest.raises(engine.MdpdError) as e:
         p.gather(j, "x")
     assert e.value.code == 2

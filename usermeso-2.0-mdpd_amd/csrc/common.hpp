@@ -21,8 +21,12 @@ constexpr int IMG_ZERO = 13;       // (0,0,0) image: (sx+1) + 3(sy+1) + 9(sz+1)
 // Consecutive entries of one particle fill whole 32-byte sectors (the row
 // builder appends a run of a particle's entries with one store), and a
 // reader loads four entries of its particle with one 16-byte load.  maxnbr
-// is always a multiple of 16.
+// is always a multiple of NBR_ALIGN = 32 = NBR_CHUNK x the largest split: the
+// first pass gives wave w of SPLIT the chunks w, w + SPLIT, ..., so a wave
+// holds at most maxnbr / SPLIT entries and its compact row t * SPLIT + w
+// stays below maxnbr.
 constexpr int NBR_CHUNK = 8;
+constexpr int NBR_ALIGN = 32;
 
 __host__ __device__ inline size_t nbr_at(int i, int q, int maxnbr)
 {

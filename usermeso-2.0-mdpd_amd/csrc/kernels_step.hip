@@ -624,6 +624,7 @@ hipError_t launch_first(const PoolDev &P, bool with_rho, int split, hipStream_t 
 {
     const int nt = P.nact / WAVE;
     if (nt == 0) return hipSuccess;
+    if (P.maxnbr % NBR_ALIGN) return hipErrorInvalidValue;   // a split wave's compact rows must fit
     switch (split) {
     case 1: launch_first_t<1>(P, with_rho, nt, st); break;
     case 2: launch_first_t<2>(P, with_rho, nt, st); break;

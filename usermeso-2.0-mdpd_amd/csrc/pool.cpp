@@ -406,7 +406,7 @@ struct mdpd_pool {
         poll_pending[0] = poll_pending[1] = false;
         for (int attempt = 0;; ++attempt) {
             if (attempt > 6) fail(MDPD_E_OVERFLOW, "<MESO> Pair table overflow; local density too high");
-            maxnbr = std::max(want, (int)std::ceil(maxnbr * 1.5 / 16.0) * 16);
+            maxnbr = std::max(want, (int)std::ceil(maxnbr * 1.5 / NBR_ALIGN) * NBR_ALIGN);
             want = 0;
             nlist.alloc((size_t)cap * maxnbr);
             clist.alloc((size_t)cap * maxnbr);
@@ -1138,7 +1138,7 @@ int mdpd_pool_stat(mdpd_pool *pool, const char *name, double *value)
         else if (k == "particle_steps") *value = pool->particle_steps;
         else if (k == "split") *value = pool->split;
         else if (k == "capacity") *value = pool->allocated ? pool->cap : (double)pool->capacity_for(pool->planned_maxnbr());
-        else if (k == "maxnbr") *value = pool->maxnbr;
+        else if (k == "maxnbr") *value = pool->allocated ? pool->maxnbr : pool->planned_maxnbr();
         else if (k == "bytes") *value = pool->allocated ? pool->device_bytes() : 0.0;
         else if (k == "regrows") *value = (double)pool->regrows;
         else if (k == "running") *value = pool->nrunning + pool->nharvest;

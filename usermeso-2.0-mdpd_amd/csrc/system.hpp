@@ -1242,7 +1242,7 @@ struct mdpd_system {
         const double dens = rho_est > 0 ? rho_est : n / ((hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]));
         const double expect = dens * 4.0 / 3.0 * M_PI * cutneigh * cutneigh * cutneigh;
         int want = std::max(32, (int)std::ceil(2.0 * expect));
-        out.want_maxnbr = (want + 15) / 16 * 16;
+        out.want_maxnbr = (want + NBR_ALIGN - 1) / NBR_ALIGN * NBR_ALIGN;
 
         // ---- Morton rank of interior cells (order of the sort key's bin part) ----
         const int nc = d.ncell;
@@ -1378,7 +1378,7 @@ struct mdpd_system {
             if (!ovf) break;
             if (attempt > 4) fail(MDPD_E_OVERFLOW, "<MESO> Pair table overflow; local density too high");
             // grow rows and rebuild the rows only (sorted data is in place)
-            maxnbr = (int)std::ceil(maxnbr * 1.5 / 16.0) * 16;
+            maxnbr = (int)std::ceil(maxnbr * 1.5 / NBR_ALIGN) * NBR_ALIGN;
             nlist.alloc((size_t)cap * maxnbr);
             clist.alloc((size_t)cap * maxnbr);
             int zero = 0;
