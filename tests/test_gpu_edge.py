@@ -1,0 +1,79 @@
+"""Edge cases of box size and particle count against the oracle.
+
+* Boxes one, two and three cells wide (cell side >= r_c + skin = 1.3): a cell
+  meets itself in its own 27-cell stencil under different periodic images, so
+  a neighbor can appear with two image codes.  Neighbor sets bit-exact,
+  densities 1e-5, forces 1e-5 (the bounds of the main parity tests).
+* One particle: its images are farther than r_c + skin, it has no neighbors,
+  no force, and flies straight (x = x0 + v t).
+* Two particles: the pair force against the oracle.
+* A partial last tile (N = 65: one full tile of 64 and one particle).
+"""
+import numpy as np
+import pytest
+
+from conftest import uniform_box
+from test_gpu_parity import IDX, RD, SEED, _neighbor_sets, assert_vec_close, mdpd_system, oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_against_oracle(engine, orc, s, L, n, what):
+    origin, _, xw = s.debug_sorted()
+    counts, rows = s.debug_neighbors()
+    got = _neighbor_sets(origin, counts, rows)
+    xc = np.empty_like(xw)
+    xc[origin] = xw
+    _, rrows = orc.neighbors(xc, (0, 0, 0), (L, L, L), 1.3)
+    repeated = 0
+    for i in range(n):
+        ref = {(int(e) & IDX, int(e) >> 27) for e in rrows[i]}
+        assert got[i] == ref, (what, i)
+        js = [j for j, _ in ref]
+        repeated += len(js) - len(set(js))
+    xg, vg = s.gather("x"), s.gather("v")
+    p = oracle_params(orc, L)
+    t0 = np.zeros(n, int)
+    rho = orc.density(xg, t0, np.ones(n, int), p, RD)
+    assert np.all(np.abs(s.gather("rho") - rho) <= 1e-5 * np.abs(rho) + 1e-12), what
+    sig = orc.sig_mdpd(np.arange(1, n + 1), orc.seed_now(SEED, 0))
+    f, _, _ = orc.force_mdpd(xg, vg, rho, sig, t0, p)
+    return f, repeated
+
+
+@pytest.mark.parametrize("L,n", [(1.5, 20), (2.0, 48), (2.7, 65), (3.9, 360)])
+def test_small_periodic_boxes_match_oracle(engine, orc, L, n):
+    x, v = uniform_box(n, L, 31)
+    s = mdpd_system(engine, x, v, L)
+    s.setup()
+    f, repeated = _check_against_oracle(engine, orc, s, L, n, f"L={L}")
+    print(f"L={L} n={n}: {repeated} neighbors listed under more than one image")
+    if L < 2.6:
+        assert repeated > 0          # the multi-image case is exercised
+    assert_vec_close(s.gather("f"), f, what=f"force L={L}")
+
+
+def test_one_particle_flies_free(engine):
+    L = 1.4
+    x0 = np.array([[0.3, 0.7, 1.1]])
+    v0 = np.array([[0.1, -0.2, 0.3]])
+    s = mdpd_system(engine, x0, v0, L)
+    s.setup()
+    counts, _ = s.debug_neighbors()
+    assert counts[0] == 0
+    s.run(5)
+    assert np.all(s.gather("f") == 0)
+    np.testing.assert_allclose(s.gather("x"), x0 + 5 * 0.01 * v0, atol=1e-6)
+    np.testing.assert_allclose(s.gather("v"), v0.astype(np.float32), atol=1e-7)
+
+
+def test_two_particles_match_oracle(engine, orc):
+    L = 2.8
+    x = np.array([[0.4, 1.0, 1.2], [0.9, 1.3, 0.9]])
+    v = np.array([[0.5, 0.1, -0.2], [-0.3, 0.2, 0.4]])
+    s = mdpd_system(engine, x, v, L)
+    s.setup()
+    f, _ = _check_against_oracle(engine, orc, s, L, 2, "two particles")
+    assert np.linalg.norm(f[0]) > 0
+    assert_vec_close(s.gather("f"), f, what="two-particle force")
+    np.testing.assert_allclose(s.gather("f").sum(axis=0), 0, atol=1e-5)
