@@ -48,8 +48,8 @@ B_STEP = B_FORCE + B_DENSITY                                 # 522 B per particl
 # HBM bytes per particle moved by the pooled force pass, measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (medians over the passes of 64
 # droplets, counters scaled by a 1 GiB copy: x2.0 fetch, x1.0 write):
-# 192.6 read + 32.1 written
-T_FORCE_POOL = 192.6 + 32.1
+# 181.4 read + 32.1 written
+T_FORCE_POOL = 181.4 + 32.1
 GB = float(1 << 30)
 
 SLABS = {
