@@ -175,3 +175,28 @@ def test_backfill_admits_past_a_job_that_does_not_fit(engine):
                p.info(j)["timestep"])
         alone = pool_results(engine, [droplet(engine, n, seed)], [k])[0][0]
         assert_same(got, alone, f"job of {n} atoms")
+
+
+def test_regrow_relists_moved_systems_from_their_last_build(engine):
+    # job a runs for a while; job b, submitted later, is far denser than the
+    # pool's rows allow, so its first build overflows and the pool widens the
+    # rows and re-lists every sorted system -- a among them, after steps taken
+    # since its last build.  a gets back the rows of its last build: its
+    # results equal a run that never overflowed.
+    a_sys = lambda: droplet(engine, 3000, 50)
+    alone = pool_results(engine, [a_sys()], [40])[0][0]
+    p = engine.Pool(0.5 * GB, 4)
+    p.tune("split", 2)
+    p.tune("poll", 4)
+    ja = p.submit(a_sys(), 40)
+    p.advance(max_passes=17)
+    assert p.state(ja) == engine.JOB_RUNNING
+    n, L = 1500, 3.4                               # density ~38
+    x, v = uniform_box(n, L, 51)
+    jb = p.submit(mdpd_system(engine, x, v, L), 3)
+    p.run()
+    assert p.stat("regrows") >= 1
+    assert p.state(jb) == engine.JOB_DONE
+    got = ({k: p.gather(ja, k) for k in ("x", "v", "f", "image", "e_pair", "virial")}, p.thermo(ja),
+           p.info(ja)["timestep"])
+    assert_same(got, alone, "job re-listed after steps")

@@ -309,8 +309,12 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 // pass append in lane order (ballot + mbcnt) to an LDS copy of the centre's
 // row, which keeps the row order; the copy is stored to the chunked row
 // layout once per centre and batch.  The workgroup is one wave.
-// relist: rows of the current, already sorted buffer (the row width grew);
-// otherwise of the data k_sortgather just wrote.
+// Rows are built from the positions of the last build (xhold), which the
+// cells were sorted by: right after the re-sort they equal the sorted
+// positions; with relist (the row width grew) every sorted system gets back
+// exactly the rows of its last build, whatever steps it has taken since, so
+// the rebuild trigger (displacement from xhold) keeps its guarantee.
+// relist selects the systems not flagged stale, otherwise the stale ones.
 // --------------------------------------------------------------------------
 constexpr int NL_ITER = 8;
 
@@ -328,7 +332,7 @@ __global__ void __launch_bounds__(WAVE, 6) k_nlist(PoolDev P, int relist, int nc
     const int c0 = P.cell_start[cs * SUBCELLS], c1 = P.cell_start[(cs + 1) * SUBCELLS];
     if (c0 >= c1) return;
     const int lane = threadIdx.x;
-    const float4 *pos = P.pos[relist ? d->sel : d->sel ^ 1];
+    const float4 *pos = P.xhold;
     // stencil table: lane k < 27 holds cell k's first particle, its length,
     // image code and shift
     int kst = 0, klen = 0;
