@@ -56,7 +56,12 @@ const char *mdpd_version(void);
  * pair_style mdpd/meso T rc seed <mobile group> <wall group>,
  * fix ID G addforce/meso fx fy fz, fix ID G rdf/fast/meso output F nbin N
  * [every M] [other G], fix ID G density/meso output F along x|y|z nbin N
- * every M window W, read_data F, write_data F, velocity G set vx vy vz,
+ * every M window W, read_data F, write_data F, write_restart F,
+ * read_restart F (checkpoint: every atom as the record of
+ * AtomVecMDPDAtomic::pack_restart, atom_vec_mdpd_atomic_meso.cu:548-570, in
+ * this library's own header with step, dt, box, masses, groups and pair
+ * settings; fixes and run_style are given again after read_restart),
+ * velocity G set vx vy vz,
  * group ID union G..., unfix, timestep, neighbor skin bin,
  * neigh_modify every/delay/check, run_style mvv/meso, reset_timestep,
  * thermo, run N.  Comments (#) and blank lines are

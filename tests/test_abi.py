@@ -284,3 +284,117 @@ def test_equal_style_variables(engine):
     with pytest.raises(engine.MdpdError) as e:
         s.command("variable s string abc")
     assert e.value.code == 6
+
+
+def _restart_source(engine):
+    s = engine.System()
+    s.command("atom_style mdpd/atomic/meso 7.5")
+    s.create_box(2, (-1, 0, 0), (4, 5, 6))
+    rng = np.random.default_rng(4)
+    x = rng.uniform((-1, 0, 0), (4, 5, 6), (30, 3))
+    x[3] = (9.5, -2.0, 6.5)                      # outside the box: write_data wraps it
+    v = rng.normal(size=(30, 3))
+    s.create_atoms(np.array([1, 2] * 15, np.int32), x, v)
+    s.commands("""
+        mass 1 1.0
+        mass 2 2.5
+        group fluid type 1
+        group wall type 2
+        pair_style mdpd/meso 1.5 1.0 777 fluid wall
+        pair_coeff 1 1 -40 25 18.0 1.0 0.75
+        pair_coeff 1 2 -20 25 18.0 1.0 0.75
+        timestep 0.02
+        reset_timestep 300
+    """)
+    return s, x, v
+
+
+def test_restart_round_trip_without_a_device(engine, tmp_path):
+    # write_restart / read_restart keep box, masses, groups, pair settings
+    # and coefficients, step, dt and the atoms; a restored system writes the
+    # same file again byte for byte
+    import struct
+    s, x, v = _restart_source(engine)
+    path = tmp_path / "a.restart"
+    s.command(f"write_restart {path}")
+    r = engine.System()
+    r.command(f"read_restart {path}")
+    assert r.natoms == 30 and r.timestep == 300
+    for name in ("tag", "type", "mask", "x", "v"):
+        np.testing.assert_array_equal(r.gather(name), s.gather(name))
+    assert (r.group_bit("fluid"), r.group_bit("wall")) == (2, 4)
+    r.command(f"write_restart {tmp_path / 'b.restart'}")
+    assert (tmp_path / "b.restart").read_bytes() == path.read_bytes()
+    s.command(f"write_data {tmp_path / 's.data'}")
+    r.command(f"write_data {tmp_path / 'r.data'}")
+    assert (tmp_path / "s.data").read_text() == (tmp_path / "r.data").read_text()
+
+    # the layout, read here on its own
+    raw = path.read_bytes()
+    assert raw[:8] == b"MDPD355R"
+    assert struct.unpack_from("<iii", raw, 8) == (1, 0x01020304, 0)   # version, byte order, mdpd/atomic/meso
+    assert struct.unpack_from("<dqd", raw, 20) == (7.5, 300, 0.02)
+    assert struct.unpack_from("<i6d", raw, 44) == (2, -1.0, 0.0, 0.0, 4.0, 5.0, 6.0)
+    assert struct.unpack_from("<idid", raw, 96) == (1, 1.0, 1, 2.5)
+    off = 120
+    groups = []
+    (ng,) = struct.unpack_from("<i", raw, off)
+    off += 4
+    for _ in range(ng):
+        bit, ln = struct.unpack_from("<ii", raw, off)
+        groups.append((raw[off + 8:off + 8 + ln].decode(), bit))
+        off += 8 + ln
+    assert groups == [("all", 1), ("fluid", 2), ("wall", 4)]
+    assert struct.unpack_from("<iddiii", raw, off) == (0, 1.5, 1.0, 777, 2, 4)
+    off += 4 + 16 + 12
+    coeffs = []
+    for _ in range(3):                          # (1,1) (1,2) (2,2)
+        (flag,) = struct.unpack_from("<i", raw, off)
+        off += 4
+        coeffs.append(struct.unpack_from("<7d", raw, off) if flag else None)
+        off += 56 if flag else 0
+    assert coeffs[0][:3] == (-40.0, 25.0, 18.0) and coeffs[0][5:] == (1.0, 0.75)
+    assert coeffs[1][0] == -20.0 and coeffs[2] is None
+    (natoms,) = struct.unpack_from("<q", raw, off)
+    off += 8
+    assert natoms == 30 and len(raw) == off + 30 * 13 * 8
+    rec = np.frombuffer(raw, np.float64, offset=off).reshape(30, 13)
+    np.testing.assert_array_equal(rec[:, 0], 13)
+    np.testing.assert_array_equal(rec[:, 1:4], x)
+    np.testing.assert_array_equal(rec[:, 4:7], v)
+    np.testing.assert_array_equal(rec[:, 7:9], 0)                   # rho, phi: not set up yet
+    np.testing.assert_array_equal(rec[:, 9], np.arange(1, 31))
+    np.testing.assert_array_equal(rec[:, 10], [1, 2] * 15)
+    np.testing.assert_array_equal(rec[:, 11], [1 | 2, 1 | 4] * 15)
+    np.testing.assert_array_equal(rec[:, 12], 512 | 512 << 10 | 512 << 20)   # image (0, 0, 0)
+
+
+def test_restart_errors(engine, tmp_path):
+    s, _, _ = _restart_source(engine)
+    path = tmp_path / "a.restart"
+    s.command(f"write_restart {path}")
+    with pytest.raises(engine.MdpdError) as e:
+        s.command(f"read_restart {path}")                  # the box is already defined
+    assert e.value.code == 2
+    raw = path.read_bytes()
+    cases = {
+        "missing.restart": None,
+        "short.restart": raw[:-8],
+        "trailing.restart": raw + b"\0",
+        "magic.restart": b"LAMMPS R" + raw[8:],
+        "type.restart": raw[:len(raw) - 30 * 104 + 80] + np.float64(3.0).tobytes() + raw[len(raw) - 30 * 104 + 88:],
+        "tag.restart": raw[:len(raw) - 30 * 104 + 72] + np.float64(0.5).tobytes() + raw[len(raw) - 30 * 104 + 80:],
+    }
+    for name, data in cases.items():
+        p = tmp_path / name
+        if data is not None:
+            p.write_bytes(data)
+        t = engine.System()
+        with pytest.raises(engine.MdpdError) as e:
+            t.command(f"read_restart {p}")
+        assert e.value.code == 1, name
+        assert t.natoms == 0, name                          # nothing installed from a bad file
+    t = engine.System()
+    with pytest.raises(engine.MdpdError) as e:
+        t.command(f"write_restart {tmp_path / 'x.restart'}")
+    assert e.value.code == 2

@@ -126,8 +126,9 @@ int mdpd_clear_atoms(mdpd_system *sys)
 {
     MDPD_GUARD(sys, {
         sys->h_tag.clear(); sys->h_type.clear(); sys->h_mask.clear(); sys->h_x.clear(); sys->h_v.clear();
-        sys->h_image.clear();
+        sys->h_image.clear(); sys->h_phi.clear();
         sys->setup_done = false;
+        sys->host_stale = false;
         sys->dirty = true;
     });
 }

@@ -123,6 +123,7 @@ struct Job {
     double mass[MAXTYPES + 1] = {0};
     std::vector<float4> hp, hv;           // packed input, npad slots
     std::vector<int> ht, hm, ho, himg;
+    std::vector<float> hph;               // wall fraction carried in (fix aboundary/rho/meso)
     // placement while admitted
     int slot = -1;
     long long part_base = -1, cell_base = -1;
@@ -334,6 +335,8 @@ struct mdpd_pool {
         j->hp.resize(j->npad); j->hv.resize(j->npad);
         j->ht.resize(j->npad); j->hm.resize(j->npad); j->ho.resize(j->npad); j->himg.resize(j->npad);
         sys->pack(j->npad, j->hp.data(), j->hv.data(), j->ht.data(), j->hm.data(), j->ho.data(), j->himg.data());
+        j->hph.assign(j->npad, 0.f);
+        for (size_t i = 0; i < sys->h_phi.size() && i < (size_t)j->n; ++i) j->hph[i] = (float)sys->h_phi[i];
         j->id = next_id++;
         if (next_id >= (1 << 27)) next_id = 1;
         j->t_submit = now_s();
@@ -486,12 +489,12 @@ struct mdpd_pool {
         const size_t ncoef = j.ds.coef.size(), nt = j.ntypes;
         // staging layout (every piece 16-byte aligned)
         auto al = [](size_t b) { return (b + 15) / 16 * 16; };
-        size_t off[13], at = 0;
-        const size_t sz[13] = {sizeof(float4) * npad, sizeof(float4) * npad, sizeof(int) * npad, sizeof(int) * npad,
+        size_t off[14], at = 0;
+        const size_t sz[14] = {sizeof(float4) * npad, sizeof(float4) * npad, sizeof(int) * npad, sizeof(int) * npad,
                                sizeof(int) * npad, sizeof(int) * npad, sizeof(int) * nc1, sizeof(int) * nc1,
                                sizeof(int) * nc1, sizeof(int) * ntw, sizeof(float) * ncoef, sizeof(float) * 2 * nt,
-                               sizeof(SysDev)};
-        for (int k = 0; k < 13; ++k) { off[k] = at; at += al(sz[k]); }
+                               sizeof(SysDev), sizeof(float) * npad};
+        for (int k = 0; k < 14; ++k) { off[k] = at; at += al(sz[k]); }
         size_t got = 0;
         char *st = (char *)pinned.get(at, got);
         memcpy(st + off[0], j.hp.data(), sz[0]);
@@ -500,6 +503,7 @@ struct mdpd_pool {
         memcpy(st + off[3], j.hm.data(), sz[3]);
         memcpy(st + off[4], j.ho.data(), sz[4]);
         memcpy(st + off[5], j.himg.data(), sz[5]);
+        memcpy(st + off[13], j.hph.data(), sz[13]);
         memcpy(st + off[6], j.ds.rank.data(), sz[6]);
         memcpy(st + off[7], j.ds.xyz.data(), sz[7]);
         int *cs = (int *)(st + off[8]);
@@ -529,7 +533,7 @@ struct mdpd_pool {
                   "upload");
         hip_check(hipMemsetAsync(cell_count.p + (size_t)cbase * SUBCELLS, 0, sizeof(int) * (size_t)nc1 * SUBCELLS, stream),
                   "memset");
-        hip_check(hipMemsetAsync(phi[0].p + pbase, 0, sizeof(float) * npad, stream), "memset");
+        up(phi[0].p + pbase, 13);
         up(dsys.p + s, 12);
         hipEvent_t e = take_event();
         hip_check(hipEventRecord(e, stream), "hipEventRecord");
@@ -550,6 +554,7 @@ struct mdpd_pool {
         std::vector<float4>().swap(j.hp); std::vector<float4>().swap(j.hv);
         std::vector<int>().swap(j.ht); std::vector<int>().swap(j.hm);
         std::vector<int>().swap(j.ho); std::vector<int>().swap(j.himg);
+        std::vector<float>().swap(j.hph);
     }
 
     // launches cover particle slots [0, nact) and cell slots [0, ncact): the

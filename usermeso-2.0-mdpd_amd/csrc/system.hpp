@@ -20,10 +20,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <fstream>
 #include <functional>
@@ -206,6 +208,8 @@ struct mdpd_system {
     // ---- atoms (creation order) ----
     std::vector<int> h_tag, h_type, h_mask, h_image;
     std::vector<double> h_x, h_v;
+    std::vector<double> h_phi;        // wall fraction carried between runs (empty: all zero)
+    bool host_stale = false;          // a run moved the device state past the host copy
 
     // ---- device state ----
     bool setup_done = false;
@@ -478,6 +482,12 @@ struct mdpd_system {
         } else if (c == "write_data") {
             need(2);
             write_data(w[1]);
+        } else if (c == "write_restart") {
+            need(2);
+            write_restart(w[1]);
+        } else if (c == "read_restart") {
+            need(2);
+            read_restart(w[1]);
         } else if (c == "velocity") {
             cmd_velocity(w);
         } else if (c == "dump") {
@@ -637,6 +647,231 @@ struct mdpd_system {
             fprintf(f, "%d %.16e %.16e %.16e\n", h_tag[i], h_v[3 * i], h_v[3 * i + 1], h_v[3 * i + 2]);
         const bool ok = fclose(f) == 0;
         if (!ok) fail(MDPD_E_ARG, "error writing data file %s", path.c_str());
+    }
+
+    // ------------------------------------------------------------------
+    // write_restart / read_restart: checkpoint of one system in this
+    // engine's own binary file (little-endian, version 1).  Each atom is the
+    // record AtomVecMDPDAtomic::pack_restart writes
+    // (atom_vec_mdpd_atomic_meso.cu:548-570), 13 doubles
+    //     [13, x, y, z, vx, vy, vz, rho, phi, tag, type, mask, image]
+    // with the integers as double values and image packed as
+    // (ix+512) | (iy+512) << 10 | (iz+512) << 20; atoms in creation order.
+    // Before the records the file holds what the LAMMPS restart header and
+    // the pair section hold for these styles (src/write_restart.cpp,
+    // MesoPairMDPD::write_restart pair_mdpd_meso.cu:403-418 and
+    // MesoPairDPD::write_restart pair_dpd_meso.cu:363-378):
+    //     char[8] "MDPD355R", int32 version, int32 0x01020304,
+    //     int32 atom_style, double rho_est, int64 ntimestep, double dt,
+    //     int32 ntypes, double lo[3], double hi[3],
+    //     ntypes x (int32 mass_set, double mass),
+    //     int32 ngroups, ngroups x (int32 bit, int32 len, char[len] name),
+    //     int32 pair_style (-1: none), double temperature, double cut_global,
+    //     int32 seed, int32 mobile_bit, int32 wall_bit,
+    //     for 1 <= i <= j <= ntypes: int32 setflag, and when set
+    //         double A, B, gamma, sigma, expw, cut, cut_r,
+    //     int64 natoms, natoms x double[13].
+    // Fixes, dumps, neighbor settings and run_style are not stored: the
+    // input script gives them again after read_restart, as in LAMMPS.
+    // rho is recomputed by the setup pass before the first force
+    // (FixArbitraryBoundaryRho::setup_pre_force, fix_aboundary_rho_meso.cu:214);
+    // phi is state the force pass reads, so it is carried over.
+    // ------------------------------------------------------------------
+    static constexpr int RESTART_VERSION = 1;
+    static constexpr int RESTART_RECORD = 13;
+
+    void write_restart(const std::string &path)
+    {
+        if (!box_set || atom_style < 0) fail(MDPD_E_STATE, "Write_restart command before simulation box is defined");
+        pull_state();
+        const size_t n = h_tag.size();
+        std::vector<double> rh(n, 0.0);
+        if (setup_done && n) gather("rho", 1, 1, rh.data());
+        FILE *f = fopen(path.c_str(), "wb");
+        if (!f) fail(MDPD_E_ARG, "Cannot open restart file %s", path.c_str());
+        bool ok = true;
+        auto put = [&](const void *p, size_t b) { ok = ok && fwrite(p, 1, b, f) == b; };
+        auto i32 = [&](int v) { put(&v, 4); };
+        auto i64 = [&](long long v) { int64_t w = v; put(&w, 8); };
+        auto f64 = [&](double v) { put(&v, 8); };
+        put("MDPD355R", 8);
+        i32(RESTART_VERSION);
+        i32(0x01020304);
+        i32(atom_style);
+        f64(rho_est);
+        i64(ntimestep);
+        f64(dt);
+        i32(ntypes);
+        for (int d = 0; d < 3; ++d) f64(lo[d]);
+        for (int d = 0; d < 3; ++d) f64(hi[d]);
+        for (int t = 1; t <= ntypes; ++t) { i32(mass_set[t] ? 1 : 0); f64(mass[t]); }
+        i32((int)groups.size());
+        for (auto &g : groups) {
+            i32(g.second);
+            i32((int)g.first.size());
+            put(g.first.data(), g.first.size());
+        }
+        i32(pair_style);
+        f64(temperature);
+        f64(cut_global);
+        i32(seed);
+        i32(pair_mobile_bit);
+        i32(pair_wall_bit);
+        for (int i = 1; i <= ntypes; ++i)
+            for (int j = i; j <= ntypes; ++j) {
+                i32(setflag[i][j]);
+                if (setflag[i][j])
+                    for (double v : {A[i][j], B[i][j], gamma[i][j], sigma[i][j], expw[i][j], cut[i][j], cut_r[i][j]})
+                        f64(v);
+            }
+        i64((long long)n);
+        std::vector<double> rec((size_t)RESTART_RECORD * n);
+        for (size_t i = 0; i < n; ++i) {
+            double *r = &rec[(size_t)RESTART_RECORD * i];
+            const int im = ((h_image[3 * i] + 512) & 1023) | (((h_image[3 * i + 1] + 512) & 1023) << 10) |
+                           (((h_image[3 * i + 2] + 512) & 1023) << 20);
+            r[0] = RESTART_RECORD;
+            for (int d = 0; d < 3; ++d) { r[1 + d] = h_x[3 * i + d]; r[4 + d] = h_v[3 * i + d]; }
+            r[7] = rh[i];
+            r[8] = i < h_phi.size() ? h_phi[i] : 0.0;
+            r[9] = h_tag[i];
+            r[10] = h_type[i];
+            r[11] = h_mask[i];
+            r[12] = im;
+        }
+        put(rec.data(), rec.size() * sizeof(double));
+        ok = (fclose(f) == 0) && ok;
+        if (!ok) fail(MDPD_E_ARG, "error writing restart file %s", path.c_str());
+    }
+
+    void read_restart(const std::string &path)
+    {
+        if (box_set) fail(MDPD_E_STATE, "Cannot read_restart after simulation box is defined");
+        FILE *f = fopen(path.c_str(), "rb");
+        if (!f) fail(MDPD_E_ARG, "Cannot open restart file %s", path.c_str());
+        std::unique_ptr<FILE, int (*)(FILE *)> hold(f, fclose);
+        auto get = [&](void *p, size_t b) {
+            if (fread(p, 1, b, f) != b) fail(MDPD_E_ARG, "Unexpected end of restart file %s", path.c_str());
+        };
+        auto i32 = [&]() { int v; get(&v, 4); return v; };
+        auto i64 = [&]() { int64_t v; get(&v, 8); return (long long)v; };
+        auto f64 = [&]() { double v; get(&v, 8); return v; };
+        char magic[8];
+        get(magic, 8);
+        if (memcmp(magic, "MDPD355R", 8) != 0) fail(MDPD_E_ARG, "%s is not a restart file of this engine", path.c_str());
+        if (i32() != RESTART_VERSION) fail(MDPD_E_UNSUPPORTED, "restart file %s has an unknown version", path.c_str());
+        if (i32() != 0x01020304) fail(MDPD_E_UNSUPPORTED, "restart file %s has the other byte order", path.c_str());
+        const int style = i32();
+        if (style != STYLE_MDPD && style != STYLE_DPD) fail(MDPD_E_ARG, "Invalid atom style in restart file");
+        const double rest = f64();
+        const long long step = i64();
+        const double tstep = f64();
+        const int nt = i32();
+        if (nt < 1 || nt > MAXTYPES) fail(MDPD_E_ARG, "number of atom types must be 1..%d", MAXTYPES);
+        double blo[3], bhi[3];
+        for (int d = 0; d < 3; ++d) blo[d] = f64();
+        for (int d = 0; d < 3; ++d) bhi[d] = f64();
+        for (int d = 0; d < 3; ++d)
+            if (!(bhi[d] > blo[d])) fail(MDPD_E_ARG, "Invalid box bounds in restart file");
+        if (!(tstep > 0) || step < 0) fail(MDPD_E_ARG, "Invalid time step in restart file");
+        int mset[MAXTYPES + 1] = {0};
+        double mval[MAXTYPES + 1] = {0};
+        for (int t = 1; t <= nt; ++t) { mset[t] = i32(); mval[t] = f64(); }
+        const int ng = i32();
+        if (ng < 1 || ng > 32) fail(MDPD_E_ARG, "Invalid group count in restart file");
+        std::vector<std::pair<std::string, int>> gs;
+        for (int k = 0; k < ng; ++k) {
+            const int bit = i32(), len = i32();
+            if (len < 1 || len > 4096 || bit != (1 << k)) fail(MDPD_E_ARG, "Invalid group in restart file");
+            std::string name((size_t)len, '\0');
+            get(&name[0], (size_t)len);
+            gs.push_back({name, bit});
+        }
+        if (gs[0].first != "all") fail(MDPD_E_ARG, "Invalid group in restart file");
+        const int ps = i32();
+        if (ps != -1 && ps != STYLE_MDPD && ps != STYLE_DPD) fail(MDPD_E_ARG, "Invalid pair style in restart file");
+        const double temp = f64(), cg = f64();
+        const int sd = i32(), mob = i32(), wal = i32();
+        const int allbits = (int)((1ull << ng) - 1);
+        if ((mob & ~allbits) || (wal & ~allbits)) fail(MDPD_E_ARG, "Invalid pair groups in restart file");
+        int sf[MAXTYPES + 1][MAXTYPES + 1] = {{0}};
+        double cv[MAXTYPES + 1][MAXTYPES + 1][7] = {{{0}}};
+        for (int i = 1; i <= nt; ++i)
+            for (int j = i; j <= nt; ++j) {
+                sf[i][j] = i32();
+                if (sf[i][j])
+                    for (int c = 0; c < 7; ++c) cv[i][j][c] = f64();
+            }
+        const long long natoms = i64();
+        if (natoms < 0 || natoms > 0x7FFFFFFF) fail(MDPD_E_ARG, "Invalid atom count in restart file");
+        const size_t n = (size_t)natoms;
+        std::vector<double> rec((size_t)RESTART_RECORD * n);
+        get(rec.data(), rec.size() * sizeof(double));
+        {
+            char extra;
+            if (fread(&extra, 1, 1, f) != 0) fail(MDPD_E_ARG, "restart file %s has trailing data", path.c_str());
+        }
+        // an integer field of a record: a whole number in int range
+        auto whole = [&](double v) {
+            if (!(v >= -2147483648.0 && v <= 2147483647.0) || v != std::floor(v))
+                fail(MDPD_E_ARG, "Invalid atom record in restart file");
+            return (int)v;
+        };
+        std::vector<int> tg(n), ty(n), mk(n), im(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            const double *r = &rec[(size_t)RESTART_RECORD * i];
+            if (r[0] != RESTART_RECORD) fail(MDPD_E_ARG, "Invalid atom record in restart file");
+            for (int c = 1; c <= 8; ++c)
+                if (!std::isfinite(r[c])) fail(MDPD_E_ARG, "Invalid atom record in restart file");
+            tg[i] = whole(r[9]);
+            ty[i] = whole(r[10]);
+            mk[i] = whole(r[11]);
+            const int pk = whole(r[12]);
+            if (tg[i] <= 0 || ty[i] < 1 || ty[i] > nt || !(mk[i] & 1) || (mk[i] & ~allbits) || pk < 0 ||
+                pk >= (1 << 30))
+                fail(MDPD_E_ARG, "Invalid atom record in restart file");
+            for (int d = 0; d < 3; ++d) im[3 * i + d] = ((pk >> (10 * d)) & 1023) - 512;
+        }
+        // everything read and checked: install it
+        atom_style = style;
+        rho_est = rest;
+        ntimestep = step;
+        dt = tstep;
+        ntypes = nt;
+        for (int d = 0; d < 3; ++d) { lo[d] = blo[d]; hi[d] = bhi[d]; }
+        box_set = true;
+        for (int t = 1; t <= nt; ++t) { mass_set[t] = mset[t] != 0; mass[t] = mval[t]; }
+        groups = gs;
+        if (ps != -1) {
+            pair_style = ps;
+            temperature = temp;
+            cut_global = cg;
+            seed = sd;
+            pair_mobile_bit = mob;
+            pair_wall_bit = wal;
+            for (int i = 1; i <= nt; ++i)
+                for (int j = i; j <= nt; ++j) {
+                    setflag[i][j] = sf[i][j];
+                    A[i][j] = cv[i][j][0]; B[i][j] = cv[i][j][1]; gamma[i][j] = cv[i][j][2];
+                    sigma[i][j] = cv[i][j][3]; expw[i][j] = cv[i][j][4]; cut[i][j] = cv[i][j][5];
+                    cut_r[i][j] = cv[i][j][6];
+                }
+        }
+        h_tag = tg;
+        h_type = ty;
+        h_mask = mk;
+        h_image = im;
+        h_x.resize(3 * n);
+        h_v.resize(3 * n);
+        h_phi.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            const double *r = &rec[(size_t)RESTART_RECORD * i];
+            for (int d = 0; d < 3; ++d) { h_x[3 * i + d] = r[1 + d]; h_v[3 * i + d] = r[4 + d]; }
+            h_phi[i] = r[8];
+        }
+        setup_done = false;
+        host_stale = false;
+        dirty = true;
     }
 
     void cmd_dump(const std::vector<std::string> &w)
@@ -1295,6 +1530,7 @@ struct mdpd_system {
 
     void setup()
     {
+        pull_state();                 // a re-setup continues from where the last run ended
         Desc ds;
         describe(ds);
         ensure_stream();
@@ -1350,7 +1586,9 @@ struct mdpd_system {
         hip_check(hipMemsetAsync(cell_count.p, 0, (size_t)nslots * SUBCELLS * sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(ncount.p, 0, cap * sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(rho.p, 0, cap * sizeof(float), stream), "memset");
-        hip_check(hipMemsetAsync(phi[0].p, 0, cap * sizeof(float), stream), "memset");
+        std::vector<float> hph(cap, 0.f);   // per-atom state of fix aboundary/rho/meso
+        for (size_t i = 0; i < h_phi.size() && i < h_tag.size(); ++i) hph[i] = (float)h_phi[i];
+        up(phi[0].p, hph.data(), cap * sizeof(float));
         hip_check(hipMemsetAsync(phi[1].p, 0, cap * sizeof(float), stream), "memset");
         hip_check(hipMemsetAsync(frc.p, 0, cap * sizeof(float4), stream), "memset");
         hip_check(hipMemsetAsync(vend.p, 0, cap * sizeof(float4), stream), "memset");
@@ -1485,6 +1723,7 @@ struct mdpd_system {
         hip_check(hipStreamSynchronize(stream), "run");
         collect_events();
         ntimestep += nsteps;
+        host_stale = true;
     }
 
     // ------------------------------------------------------------------
@@ -1581,16 +1820,21 @@ struct mdpd_system {
     }
 
     // pull the device state back into the creation-order host arrays
+    // (only when a run has advanced the device past the host copy, so that
+    // host edits made since, such as scattered positions, are kept)
     void pull_state()
     {
-        if (!setup_done) return;
-        std::vector<double> x(3 * h_tag.size()), v(3 * h_tag.size());
+        if (!setup_done || !host_stale) return;
+        std::vector<double> x(3 * h_tag.size()), v(3 * h_tag.size()), ph(h_tag.size());
         std::vector<int> im(3 * h_tag.size());
         gather("x", 1, 3, x.data());
         gather("v", 1, 3, v.data());
         gather("image", 0, 3, im.data());
+        gather("phi", 1, 1, ph.data());
         h_x = x;
         h_v = v;
         h_image = im;
+        h_phi = ph;
+        host_stale = false;
     }
 };
