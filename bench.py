@@ -17,7 +17,8 @@ The served leg (`--served-seconds`, 0 to skip) submits jobs with Poisson
 arrivals (`--served-rate` per second): each a droplet or a slab (50/50), N
 log-uniform in [1000, 200000], `--steps` steps, admitted by the pool's HBM
 budget while others run; it reports throughput and p50/p99 latency from
-arrival to result.
+arrival to result.  The jobs' synthetic particles are made before the clock
+starts (client data); building each system and submitting it is timed.
 
 Other workloads on the plain path (one handle): `slab` (box 40 x 80 x 20,
 slab z in [5, 15) at density 6.05, N = 193,600, Lucy weight; `slab-literal`:
@@ -240,7 +241,7 @@ def batched(specs, steps, warmup, split, budget, graph=True):
 # ---------------------------------------------------------------------------
 # served: Poisson arrivals into a pool with an HBM budget
 # ---------------------------------------------------------------------------
-def served(seconds, rate, steps, budget, seed, max_wall=240.0):
+def served(seconds, rate, steps, budget, seed, max_wall=240.0, split=0):
     import mdpd355
     rng = np.random.default_rng(seed)
     arr = []
@@ -251,22 +252,32 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0):
     kinds = rng.random(len(arr)) < 0.5
     sizes = np.round(np.exp(rng.uniform(np.log(1000.0), np.log(200000.0), len(arr)))).astype(int)
     spec = lambda i: (make_droplet(sizes[i], 7000 + i) if kinds[i] else make_slab_job(sizes[i], 7000 + i))
+    # the clients' inputs exist before they arrive: the synthetic particles
+    # are made before the clock starts; building the system and submitting
+    # it (the engine's API) is timed
+    inputs = [spec(i) for i in range(len(arr))]
     p = mdpd355.Pool(budget, 1024)
     p.tune("poll", 4)
+    p.tune("profile", 16)
+    p.tune("split", split)                     # 0: autotuned
     meta, lat, work = {}, [], 0.0
     occ = []                                   # (used slot fraction, running jobs, queued jobs) per loop
+    t_adv = 0.0                                # wall time inside advance()
     t0 = mdpd355.clock()
     i = 0
     while True:
         now = mdpd355.clock() - t0
         while i < len(arr) and arr[i] <= now:
-            box, x, v = spec(i)
+            box, x, v = inputs[i]
+            inputs[i] = None
             s = system(box, x, v, 9000 + i)
             meta[p.submit(s, steps)] = (t0 + arr[i], len(x))
             s.close()
             i += 1
         if p.stat("running") + p.stat("queued") > 0:
+            ta = mdpd355.clock()
             p.advance(max_passes=8)
+            t_adv += mdpd355.clock() - ta
             occ.append((p.stat("used_slots") / max(p.stat("capacity"), 1.0), p.stat("running"), p.stat("queued")))
         elif i < len(arr):
             time.sleep(min(0.002, max(0.0, arr[i] - now)))
@@ -281,8 +292,11 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0):
         if mdpd355.clock() - t0 > max_wall:
             break
     wall = mdpd355.clock() - t0
+    st = {k: p.stat(k) for k in ("passes", "particle_steps", "captures", "split", "prof_passes", "prof_rebuild_ms",
+                                 "prof_first_ms", "prof_force_ms", "prof_commit_ms")}
     p.close()
     lat = np.array(lat)
+    npp = max(st["prof_passes"], 1.0)
     out = {
         "arrivals": len(arr), "completed": int(len(lat)), "window_s": seconds, "rate_per_s": rate,
         "steps_per_job": steps, "budget_GB": budget / GB, "wall_s": wall,
@@ -293,6 +307,11 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0):
         "mean_used_slot_fraction": float(np.mean([o[0] for o in occ])) if occ else None,
         "mean_running_jobs": float(np.mean([o[1] for o in occ])) if occ else None,
         "mean_queued_jobs": float(np.mean([o[2] for o in occ])) if occ else None,
+        "advance_wall_fraction": t_adv / wall if wall > 0 else None,
+        "passes": st["passes"], "ms_per_pass": wall * 1e3 / max(st["passes"], 1.0),
+        "mean_particles_per_pass": st["particle_steps"] / max(st["passes"], 1.0),
+        "graph_captures": st["captures"], "split": st["split"],
+        "sampled_ms_per_pass": {k: st[f"prof_{k}_ms"] / npp for k in ("rebuild", "first", "force", "commit")},
     }
     return out, sizes, (lambda i: (*spec(i), 9000 + i))
 
@@ -352,7 +371,8 @@ def run_batched(a):
                                       if a.plain_sample > 0 else (None, [], 0.0))
     srv = None
     if a.served_seconds > 0:
-        srv, ssizes, sspec = served(a.served_seconds, a.served_rate, a.steps, a.served_budget_gb * GB, a.seed + 99)
+        srv, ssizes, sspec = served(a.served_seconds, a.served_rate, a.steps, a.served_budget_gb * GB, a.seed + 99,
+                                    split=a.served_split)
         if a.plain_sample > 0:
             sv, sns, ss = plain_sample(ssizes, sspec, a.steps, max(4, a.plain_sample // 2))
             srv.update({"plain_value": sv, "plain_sample_sizes": sns, "served_vs_plain": srv["value"] / sv})
@@ -548,6 +568,8 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=10.0, help="arrival window of the served leg (0: skip)")
     ap.add_argument("--served-rate", type=float, default=50.0)
     ap.add_argument("--served-budget-gb", type=float, default=4.0)
+    ap.add_argument("--served-split", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="waves per tile of the served pool (0: autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched leg: launch each pass's kernels directly instead of replaying a hipGraph")

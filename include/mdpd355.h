@@ -224,8 +224,9 @@ int mdpd_pool_thermo(mdpd_pool *pool, int job, double out[4]);
  * mdpd_clock) of a job. */
 int mdpd_pool_job_info(mdpd_pool *pool, int job, long long *natoms, long long *timestep, double times[3]);
 int mdpd_pool_release(mdpd_pool *pool, int job);
-/* "split" 0 (autotune on the first passes and once more after 256 passes,
- * the default), 1, 2 or 4;
+/* "split" 0 (autotune on the first passes, once more after 256 passes and
+ * whenever the resident load has doubled or halved since, at most every 256
+ * passes; the default), 1, 2 or 4;
  * "graph" 1/0: replay each pass as a captured hipGraph (default 1);
  * "poll": passes between status reads (default 8);
  * "profile" N: every N-th batch of passes is launched without the graph with

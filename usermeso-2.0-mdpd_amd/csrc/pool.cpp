@@ -185,11 +185,13 @@ struct mdpd_pool {
     // ---- launch shape ----
     int split = 2;
     bool autotune = true, tuned = false, retuned = false;
-    long long retune_at = 0;              // passes after which the split is tuned once more
+    long long retune_at = 0;              // passes after which the split may be tuned again
+    double tuned_load = 0;                // resident particle slots when the split was last tuned
     double tune_ms[5] = {0, 0, 0, 0, 0};
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
+    long long captures = 0;           // graphs of the pass captured so far
     int passes_per_poll = 8;
     std::set<std::pair<int, int>> combos;  // (style, single type) of admitted jobs
     bool with_rho = false, with_arb = false;
@@ -667,6 +669,7 @@ struct mdpd_pool {
         (void)hipGraphDestroy(g);
         hip_check(e, "hipGraphInstantiate");
         graph_valid = true;
+        ++captures;
     }
 
     void run_passes(int m)
@@ -694,10 +697,18 @@ struct mdpd_pool {
     // summation order.  Only the split-dependent part of a pass is compared
     // (first pass + force pass): the rebuild kernels do not depend on the
     // split, and the share of systems re-sorting varies from pass to pass.
+    // the resident load moved by 2x either way since the split was tuned
+    bool load_moved() const
+    {
+        const double l = (double)parts.used;
+        return l > 2.0 * tuned_load || l < 0.5 * tuned_load;
+    }
+
     void tune()
     {
         tuned = true;
         if (!autotune) return;
+        tuned_load = (double)parts.used;
         const int cand[3] = {1, 2, 4};
         double ms[3] = {0, 0, 0};
         std::vector<PassEvents> evs;
@@ -916,8 +927,11 @@ struct mdpd_pool {
             if (max_passes > 0 && launched >= max_passes) break;
             // the first passes after a start from raw input are not typical
             // (rows relax, most systems re-sort every pass): the split is
-            // tuned again once the pool has run RETUNE_PASSES passes
-            if (!tuned || (autotune && !retuned && passes >= retune_at)) {
+            // tuned again once the pool has run RETUNE_PASSES passes, and
+            // after that whenever the resident load has doubled or halved
+            // (a served pool filling up or draining), at most once every
+            // RETUNE_PASSES passes
+            if (!tuned || (autotune && passes >= retune_at && (!retuned || load_moved()))) {
                 if (tuned) retuned = true;
                 tune();
                 retune_at = passes + RETUNE_PASSES;
@@ -1146,6 +1160,7 @@ int mdpd_pool_stat(mdpd_pool *pool, const char *name, double *value)
         else if (k == "maxnbr") *value = pool->allocated ? pool->maxnbr : pool->planned_maxnbr();
         else if (k == "bytes") *value = pool->allocated ? pool->device_bytes() : 0.0;
         else if (k == "regrows") *value = (double)pool->regrows;
+        else if (k == "captures") *value = (double)pool->captures;
         else if (k == "running") *value = pool->nrunning + pool->nharvest;
         else if (k == "queued") *value = (double)pool->queue.size();
         else if (k == "used_slots") *value = (double)pool->parts.used * 256.0;
