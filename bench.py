@@ -241,7 +241,7 @@ def batched(specs, steps, warmup, split, budget, graph=True):
 # ---------------------------------------------------------------------------
 # served: Poisson arrivals into a pool with an HBM budget
 # ---------------------------------------------------------------------------
-def served(seconds, rate, steps, budget, seed, max_wall=240.0, split=0):
+def served(seconds, rate, steps, budget, seed, max_wall=240.0, split=0, graph=True):
     import mdpd355
     rng = np.random.default_rng(seed)
     arr = []
@@ -260,6 +260,8 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0, split=0):
     p.tune("poll", 4)
     p.tune("profile", 16)
     p.tune("split", split)                     # 0: autotuned
+    if not graph:
+        p.tune("graph", 0)
     meta, lat, work = {}, [], 0.0
     occ = []                                   # (used slot fraction, running jobs, queued jobs) per loop
     t_adv = 0.0                                # wall time inside advance()
@@ -372,7 +374,7 @@ def run_batched(a):
     srv = None
     if a.served_seconds > 0:
         srv, ssizes, sspec = served(a.served_seconds, a.served_rate, a.steps, a.served_budget_gb * GB, a.seed + 99,
-                                    split=a.served_split)
+                                    split=a.served_split, graph=not a.no_graph)
         if a.plain_sample > 0:
             sv, sns, ss = plain_sample(ssizes, sspec, a.steps, max(4, a.plain_sample // 2))
             srv.update({"plain_value": sv, "plain_sample_sizes": sns, "served_vs_plain": srv["value"] / sv})
@@ -572,7 +574,7 @@ def main():
                     help="waves per tile of the served pool (0: autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true",
-                    help="batched leg: launch each pass's kernels directly instead of replaying a hipGraph")
+                    help="batched and served pools: launch each pass's kernels directly instead of replaying a hipGraph")
     ap.add_argument("--split", type=int, default=None, help="waves per tile (default: autotuned / 2)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device with a plain copy kernel (a known-size "

@@ -157,7 +157,8 @@ struct PoolDev {
     int nact;            // slots [0, nact) hold systems: extent of the launches (<= cap)
     int ncellslots;
     int ncact;           // cell slots [0, ncact) hold systems: extent of per-cell launches
-    int nsys;
+    int nsys;            // descriptor slots [0, nsys) hold systems: extent of per-system launches
+    int maxsub;          // sub-cell slots of the largest system (0: all cell slots, one system)
     int maxnbr;
 };
 

@@ -498,10 +498,11 @@ hipError_t launch_rebuild(const PoolDev &P, hipStream_t st)
     if (P.nact == 0) return hipSuccess;
     const dim3 g(nblk(P.nact, 256)), b(256);
     hipLaunchKernelGGL(k_keys, g, b, 0, st, P);
-    // workgroups per system for the chunked scan: one system spreads over up
-    // to 64, a pool of many systems runs them side by side
-    const int nch = (int)(((long long)P.ncellslots * SUBCELLS + SCAN_CH - 1) / SCAN_CH);
-    const int gs = P.nsys == 1 ? std::max(1, std::min(64, nch)) : 4;
+    // workgroups per system for the chunked scan: as many as the largest
+    // system has chunks, up to 64 (smaller systems' extra workgroups exit)
+    const long long nsub = P.maxsub > 0 ? (long long)P.maxsub : (long long)P.ncellslots * SUBCELLS;
+    const int nch = (int)((nsub + SCAN_CH - 1) / SCAN_CH);
+    const int gs = std::max(1, std::min(64, nch));
     hipLaunchKernelGGL(k_scan_sums, dim3(gs, P.nsys), dim3(SCAN_BLK), 0, st, P);
     hipLaunchKernelGGL(k_scan_chunks, dim3(P.nsys), dim3(SCAN_BLK), 0, st, P);
     hipLaunchKernelGGL(k_scan_apply, dim3(gs, P.nsys), dim3(SCAN_BLK), 0, st, P);

@@ -191,6 +191,8 @@ struct mdpd_pool {
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
+    int nsys_act = 64;                    // per-system launch extent (descriptor slots)
+    int maxsub = 16384;                   // sub-cell slots of the largest resident system
     long long captures = 0;           // graphs of the pass captured so far
     int passes_per_poll = 8;
     std::set<std::pair<int, int>> combos;  // (style, single type) of admitted jobs
@@ -287,7 +289,8 @@ struct mdpd_pool {
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
         P.cl = cl.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
-        P.cap = cap; P.nact = nact; P.ncellslots = nslots; P.ncact = ncact; P.nsys = max_sys; P.maxnbr = maxnbr;
+        P.cap = cap; P.nact = nact; P.ncellslots = nslots; P.ncact = ncact; P.nsys = nsys_act; P.maxsub = maxsub;
+        P.maxnbr = maxnbr;
         return P;
     }
 
@@ -562,6 +565,10 @@ struct mdpd_pool {
     // launches cover particle slots [0, nact) and cell slots [0, ncact): the
     // highest resident ranges, rounded up to 64K slots so the captured pass is
     // rebuilt only when they move by that
+    // Per-system launches cover descriptor slots [0, nsys_act): the highest
+    // slot holding a job, rounded up to 64 (free slots are reused last in,
+    // first out, so the occupied ones stay low); the chunked scan takes as
+    // many workgroups per system as the largest resident system needs.
     void update_extent()
     {
         const long long hw = parts.high() * 256;
@@ -569,9 +576,24 @@ struct mdpd_pool {
         const int want = (int)std::min<long long>(cap, (hw + q - 1) / q * q);
         const long long hc = cells.high();
         const int wantc = (int)std::min<long long>(nslots, (hc + q - 1) / q * q);
-        if (want != nact || wantc != ncact) {
+        int hs = 0;
+        long long mc = 0;
+        for (int s = 0; s < max_sys; ++s)
+            if (slot_job[s]) {
+                hs = s + 1;
+                const auto it = jobs.find(slot_job[s]);
+                if (it != jobs.end()) mc = std::max<long long>(mc, it->second->ncell + 1);
+            }
+        const int wants = std::min(max_sys, std::max(64, (hs + 63) / 64 * 64));
+        // sub-cell slots of the largest system, rounded to 16K so that the
+        // captured pass changes only when it moves by that much
+        const long long qs = 16384;
+        const int wantm = (int)std::max<long long>(qs, (mc * SUBCELLS + qs - 1) / qs * qs);
+        if (want != nact || wantc != ncact || wants != nsys_act || wantm != maxsub) {
             nact = want;
             ncact = wantc;
+            nsys_act = wants;
+            maxsub = wantm;
             graph_valid = false;
         }
     }

@@ -1367,7 +1367,8 @@ struct mdpd_system {
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
         P.cl = cl.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
-        P.cap = cap; P.nact = cap; P.ncellslots = nslots; P.ncact = nslots; P.nsys = 1; P.maxnbr = maxnbr;
+        P.cap = cap; P.nact = cap; P.ncellslots = nslots; P.ncact = nslots; P.nsys = 1; P.maxsub = 0;
+        P.maxnbr = maxnbr;
         return P;
     }
 
