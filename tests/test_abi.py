@@ -384,6 +384,8 @@ def test_restart_errors(engine, tmp_path):
         "magic.restart": b"LAMMPS R" + raw[8:],
         "type.restart": raw[:len(raw) - 30 * 104 + 80] + np.float64(3.0).tobytes() + raw[len(raw) - 30 * 104 + 88:],
         "tag.restart": raw[:len(raw) - 30 * 104 + 72] + np.float64(0.5).tobytes() + raw[len(raw) - 30 * 104 + 80:],
+        # an atom count far beyond the records that follow it
+        "count.restart": raw[:len(raw) - 30 * 104 - 8] + np.int64(1 << 30).tobytes() + raw[len(raw) - 30 * 104:],
     }
     for name, data in cases.items():
         p = tmp_path / name

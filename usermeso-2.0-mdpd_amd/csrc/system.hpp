@@ -804,6 +804,16 @@ struct mdpd_system {
             }
         const long long natoms = i64();
         if (natoms < 0 || natoms > 0x7FFFFFFF) fail(MDPD_E_ARG, "Invalid atom count in restart file");
+        {
+            // the records must be exactly what is left of the file
+            const long at = ftell(f);
+            if (at < 0 || fseek(f, 0, SEEK_END) != 0) fail(MDPD_E_ARG, "Cannot read restart file %s", path.c_str());
+            const long end = ftell(f);
+            if (end < 0 || fseek(f, at, SEEK_SET) != 0) fail(MDPD_E_ARG, "Cannot read restart file %s", path.c_str());
+            if ((long long)(end - at) != natoms * (long long)(RESTART_RECORD * sizeof(double)))
+                fail(MDPD_E_ARG, "restart file %s holds %ld bytes of atom records for %lld atoms", path.c_str(),
+                     end - at, natoms);
+        }
         const size_t n = (size_t)natoms;
         std::vector<double> rec((size_t)RESTART_RECORD * n);
         get(rec.data(), rec.size() * sizeof(double));
