@@ -246,17 +246,26 @@ __device__ inline void pair_term(Acc<EV> &a, const float4 &pi, const float4 &vi,
     const float dot = dx * dvx + dy * dvy + dz * dvz;
     const float wc = 1.f - r * c.rcinv;
     const float xi = pair_gaussian(__float_as_uint(vi.w), __float_as_uint(vj.w));
+    // sigma / sqrt(dt): uniform for a one-type system away from walls
+    const float sdt = c.sigma * dtis;
     float fpair, en = 0.f;
     if (STYLE == STYLE_MDPD) {
-        // pair_mdpd_meso.cu:156-191
+        // pair_mdpd_meso.cu:156-191: F = [A wc + B (rho_i + rho_j) wr
+        //   - gamma wc^2 (dx.dv)/r + sigma wc xi / sqrt(dt)] / r, evaluated as
+        // [wc (A + sigma' xi - gamma wc (dx.dv)/r) + B (rho_i + rho_j) wr] / r
         const float wcr = fmaxf(1.f - r * c.rdinv, 0.f);
         const float rr = rho_i + rho_j;
-        fpair = ((c.a * wc + c.b * rr * wcr) - c.gamma * wc * wc * dot * rinv + c.sigma * wc * xi * dtis) * rinv;
+        float t = fmaf(sdt, xi, c.a);
+        t = fmaf(-(c.gamma * wc * rinv), dot, t);
+        fpair = fmaf(c.b * rr, wcr, t * wc) * rinv;
         if (EV) en = 0.5f * c.a * c.rc * wc * wc + 0.5f * c.b * c.rd * rr * wcr * wcr;
     } else {
-        // pair_dpd_meso.cu:145-175
+        // pair_dpd_meso.cu:145-175: F = [a0 wc - gamma wr^2 (dx.dv)/r
+        //   + sigma wr xi / sqrt(dt)] / r, evaluated as
+        // [a0 wc + wr (sigma' xi - gamma wr (dx.dv)/r)] / r
         const float wr = c.expw == 1.f ? wc : powf(wc, c.expw);
-        fpair = (c.a * wc - c.gamma * wr * wr * dot * rinv + c.sigma * wr * xi * dtis) * rinv;
+        const float t = fmaf(-(c.gamma * wr * rinv), dot, sdt * xi);
+        fpair = fmaf(c.a, wc, wr * t) * rinv;
         if (EV) en = 0.5f * c.a * c.rc * wc * wc;
     }
     fpair = in ? fpair : 0.f;
