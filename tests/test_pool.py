@@ -200,3 +200,16 @@ def test_regrow_relists_moved_systems_from_their_last_build(engine):
     got = ({k: p.gather(ja, k) for k in ("x", "v", "f", "image", "e_pair", "virial")}, p.thermo(ja),
            p.info(ja)["timestep"])
     assert_same(got, alone, "job re-listed after steps")
+
+
+def test_many_resident_jobs_beyond_one_launch_block(engine):
+    # 80 resident jobs: per-system launches cover the occupied descriptor
+    # slots rounded up to 64, so slots 64..79 sit in the second block; jobs
+    # on both sides equal their plain runs, and the sizes differ so the
+    # chunked scan runs with several workgroups per system
+    sizes = [400 + 37 * k for k in range(79)] + [20000]
+    mk = lambda k: droplet(engine, sizes[k], 300 + k)
+    nsteps = [6 + k % 5 for k in range(80)]
+    got, p = pool_results(engine, [mk(k) for k in range(80)], nsteps, budget=1.0 * GB, max_systems=1024)
+    for k in (0, 63, 64, 78, 79):
+        assert_same(got[k], plain_result(mk(k), nsteps[k]), f"job {k}")
