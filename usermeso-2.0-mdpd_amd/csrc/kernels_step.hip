@@ -116,27 +116,48 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
     // gathers in flight per lane, the next chunk's entries loading meanwhile;
     // wave w takes chunks w, w + SPLIT, ... and visits them in row order.
     // Entries past the row's end are neither gathered nor visited.
+    // With one wave per tile (SPLIT 1) the LDS staging, not registers, limits
+    // occupancy, so a wave takes U = 2 chunks per iteration: sixteen gathers
+    // in flight, the next two chunks' entries loading meanwhile.
+    constexpr int U = SPLIT == 1 ? 2 : 1;
     const int nch = (cnt + NBR_CHUNK - 1) / NBR_CHUNK;
     const int4 *row4 = (const int4 *)row;
-    int4 ea = make_int4(0, 0, 0, 0), eb = ea;
-    if (w < nch) {
-        ea = row4[(size_t)w * (NBR_CHUNK * WAVE / 4)];
-        eb = row4[(size_t)w * (NBR_CHUNK * WAVE / 4) + 1];
-    }
-    for (int c = w; c < nch; c += SPLIT) {
-        const int p = c * NBR_CHUNK;
-        const uint32_t e[NBR_CHUNK] = {(uint32_t)ea.x, (uint32_t)ea.y, (uint32_t)ea.z, (uint32_t)ea.w,
-                                       (uint32_t)eb.x, (uint32_t)eb.y, (uint32_t)eb.z, (uint32_t)eb.w};
-        if (c + SPLIT < nch) {
-            ea = row4[(size_t)(c + SPLIT) * (NBR_CHUNK * WAVE / 4)];
-            eb = row4[(size_t)(c + SPLIT) * (NBR_CHUNK * WAVE / 4) + 1];
+    int4 ea[U], eb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        ea[u] = make_int4(0, 0, 0, 0);
+        eb[u] = ea[u];
+        const int cu = w + u * SPLIT;
+        if (cu < nch) {
+            ea[u] = row4[(size_t)cu * (NBR_CHUNK * WAVE / 4)];
+            eb[u] = row4[(size_t)cu * (NBR_CHUNK * WAVE / 4) + 1];
         }
-        float4 q[NBR_CHUNK];
+    }
+    for (int c = w; c < nch; c += SPLIT * U) {
+        uint32_t e[U * NBR_CHUNK];
 #pragma unroll
-        for (int u = 0; u < NBR_CHUNK; ++u) q[u] = pos[(p + u < cnt ? e[u] : (uint32_t)i) & IDX_MASK];
+        for (int u = 0; u < U; ++u) {
+            e[u * NBR_CHUNK + 0] = (uint32_t)ea[u].x; e[u * NBR_CHUNK + 1] = (uint32_t)ea[u].y;
+            e[u * NBR_CHUNK + 2] = (uint32_t)ea[u].z; e[u * NBR_CHUNK + 3] = (uint32_t)ea[u].w;
+            e[u * NBR_CHUNK + 4] = (uint32_t)eb[u].x; e[u * NBR_CHUNK + 5] = (uint32_t)eb[u].y;
+            e[u * NBR_CHUNK + 6] = (uint32_t)eb[u].z; e[u * NBR_CHUNK + 7] = (uint32_t)eb[u].w;
+            const int cn = c + (U + u) * SPLIT;
+            if (cn < nch) {
+                ea[u] = row4[(size_t)cn * (NBR_CHUNK * WAVE / 4)];
+                eb[u] = row4[(size_t)cn * (NBR_CHUNK * WAVE / 4) + 1];
+            }
+        }
+        float4 q[U * NBR_CHUNK];
 #pragma unroll
-        for (int u = 0; u < NBR_CHUNK; ++u)
-            if (p + u < cnt) visit(e[u], q[u]);
+        for (int k = 0; k < U * NBR_CHUNK; ++k) {
+            const int p = (c + (k / NBR_CHUNK) * SPLIT) * NBR_CHUNK + k % NBR_CHUNK;
+            q[k] = pos[(p < cnt ? e[k] : (uint32_t)i) & IDX_MASK];
+        }
+#pragma unroll
+        for (int k = 0; k < U * NBR_CHUNK; ++k) {
+            const int p = (c + (k / NBR_CHUNK) * SPLIT) * NBR_CHUNK + k % NBR_CHUNK;
+            if (p < cnt) visit(e[k], q[k]);
+        }
     }
     // coalesced store of the staged rows
     int ncmax = min(nc, STAGE);
