@@ -358,9 +358,6 @@ __global__ void __launch_bounds__(WAVE, 6) k_nlist(PoolDev P, int relist, int nc
     const int kend = wave_incl_scan(klen, lane);
     const int kpre = kend - klen;                           // candidates before cell k
     const int ntot = __builtin_amdgcn_readlane(kend, 26);
-    // the centre's own cell is stencil cell 13 (no image shift): centre c is
-    // candidate pre13 + (c - c0) of its own list
-    const int pre13 = __builtin_amdgcn_readlane(kpre, 13);
     const float rc2 = d->rc2_list;
     const int maxnbr = P.maxnbr;
     for (int cg = c0; cg < c1; cg += WAVE) {
@@ -402,16 +399,13 @@ __global__ void __launch_bounds__(WAVE, 6) k_nlist(PoolDev P, int relist, int nc
                 const float pcy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pg.y), cl));
                 const float pcz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pg.z), cl));
                 const int q0 = __builtin_amdgcn_readlane(cntv, cl);     // row length before this batch
-                const int tself = pre13 + (c - c0);
-                int n = 0;                                  // entries of this batch, staged in LDS
+                int n = 0;                                  // hits of this batch, staged in LDS
 #pragma unroll
                 for (int m = 0; m < NL_ITER; ++m) {
                     if (m >= nm) break;
                     const float dx = pcx - qx[m], dy = pcy - qy[m], dz = pcz - qz[m];
                     const float r2 = (dx * dx + dy * dy) + dz * dz;
-                    uint64_t mask = __builtin_amdgcn_ballot_w64(r2 <= rc2);
-                    const int sl = tself - (tb + m * WAVE);  // the centre itself: never listed
-                    if ((unsigned)sl < (unsigned)WAVE) mask &= ~(1ull << sl);
+                    const uint64_t mask = __builtin_amdgcn_ballot_w64(r2 <= rc2);
                     if (mask) {
                         if (__builtin_amdgcn_inverse_ballot_w64(mask))
                             s_row[n + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
@@ -419,18 +413,28 @@ __global__ void __launch_bounds__(WAVE, 6) k_nlist(PoolDev P, int relist, int nc
                         n += __popcll(mask);
                     }
                 }
-                // this batch's entries of centre c: rows [q0, q0 + n) in the
+                // this batch's entries of centre c: rows [q0, ...) in the
                 // chunked layout, entries past the row width dropped (the
-                // count marks the overflow)
+                // count marks the overflow).  The centre itself is among the
+                // hits once in its whole row (stencil cell 13, the only one
+                // with a zero image shift, distance 0): that entry is left
+                // out and the entries after it move up by one.
                 __syncthreads();
+                const int selfe = (int)((uint32_t)c | (IMG_ZERO << IMG_SHIFT));
                 int *row = P.nlist + nbr_at(c, 0, maxnbr);
-                for (int k = lane; k < n; k += WAVE) {
-                    const uint32_t q = (uint32_t)(q0 + k);
-                    if (q < (uint32_t)maxnbr)
-                        row[(q & ~(uint32_t)(NBR_CHUNK - 1)) * WAVE + (q & (uint32_t)(NBR_CHUNK - 1))] = s_row[k];
+                int shift = 0;
+                for (int k0 = 0; k0 < n; k0 += WAVE) {
+                    const int k = k0 + lane;
+                    const int v = k < n ? s_row[k] : 0;
+                    const uint64_t sb = __builtin_amdgcn_ballot_w64(k < n && v == selfe);
+                    const int ps = sb ? (int)__builtin_ctzll(sb) : WAVE;   // lane holding the centre
+                    const uint32_t q = (uint32_t)(q0 + k - shift - (lane > ps ? 1 : 0));
+                    if (k < n && lane != ps && q < (uint32_t)maxnbr)
+                        row[(q & ~(uint32_t)(NBR_CHUNK - 1)) * WAVE + (q & (uint32_t)(NBR_CHUNK - 1))] = v;
+                    if (sb) shift = 1;
                 }
                 __syncthreads();
-                if (lane == cl) cntv = q0 + n;
+                if (lane == cl) cntv = q0 + n - shift;
             }
         }
         if (lane < ng) P.ncount[cg + lane] = cntv;
