@@ -21,6 +21,8 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace mdpd {
 
 constexpr int BLK = 256;
@@ -99,7 +101,8 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
     const bool lucy = d->lucy;
     float sum = 0.f;
     int nc = 0;
-    auto visit = [&](uint32_t e, const float4 &q) {
+    // ROOM: the staging area is known to hold this entry (no spill check)
+    auto visit = [&](uint32_t e, const float4 &q, auto room) {
         const float rsq = pair_rsq(pi, q, e >> IMG_SHIFT, d);
         if (WITH_RHO && rsq < h2 && rsq >= 1e-20f) {
             const float r = __builtin_amdgcn_sqrtf(rsq);
@@ -107,8 +110,12 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
             sum += lucy ? (h + 3.f * r) * hm * hm * hm : hm * hm;
         }
         if (rsq < rc2) {
-            if (nc < STAGE) s_c[w][nc * WAVE + lane] = (int)e;
-            else crow[(size_t)(nc * SPLIT + w) * WAVE] = (int)e;
+            if constexpr (decltype(room)::value) {
+                s_c[w][nc * WAVE + lane] = (int)e;
+            } else {
+                if (nc < STAGE) s_c[w][nc * WAVE + lane] = (int)e;
+                else crow[(size_t)(nc * SPLIT + w) * WAVE] = (int)e;
+            }
             ++nc;
         }
     };
@@ -148,15 +155,26 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
             }
         }
         float4 q[U * NBR_CHUNK];
+        // when every active lane's row covers all of this iteration's
+        // entries and its staging area has room for all of them, the
+        // per-entry row-end and staging checks are skipped (same result)
+        const bool full = (c + (U - 1) * SPLIT + 1) * NBR_CHUNK <= cnt && nc + U * NBR_CHUNK <= STAGE;
+        if (__builtin_amdgcn_ballot_w64(!full) == 0) {
 #pragma unroll
-        for (int k = 0; k < U * NBR_CHUNK; ++k) {
-            const int p = (c + (k / NBR_CHUNK) * SPLIT) * NBR_CHUNK + k % NBR_CHUNK;
-            q[k] = pos[(p < cnt ? e[k] : (uint32_t)i) & IDX_MASK];
-        }
+            for (int k = 0; k < U * NBR_CHUNK; ++k) q[k] = pos[e[k] & IDX_MASK];
 #pragma unroll
-        for (int k = 0; k < U * NBR_CHUNK; ++k) {
-            const int p = (c + (k / NBR_CHUNK) * SPLIT) * NBR_CHUNK + k % NBR_CHUNK;
-            if (p < cnt) visit(e[k], q[k]);
+            for (int k = 0; k < U * NBR_CHUNK; ++k) visit(e[k], q[k], std::true_type{});
+        } else {
+#pragma unroll
+            for (int k = 0; k < U * NBR_CHUNK; ++k) {
+                const int p = (c + (k / NBR_CHUNK) * SPLIT) * NBR_CHUNK + k % NBR_CHUNK;
+                q[k] = pos[(p < cnt ? e[k] : (uint32_t)i) & IDX_MASK];
+            }
+#pragma unroll
+            for (int k = 0; k < U * NBR_CHUNK; ++k) {
+                const int p = (c + (k / NBR_CHUNK) * SPLIT) * NBR_CHUNK + k % NBR_CHUNK;
+                if (p < cnt) visit(e[k], q[k], std::false_type{});
+            }
         }
     }
     // coalesced store of the staged rows
