@@ -236,25 +236,31 @@ __global__ void __launch_bounds__(256) k_scatter(PoolDev P)
 }
 
 // --------------------------------------------------------------------------
-// K4: one thread per particle, by its arrival slot f in its sub-cell's
-// segment: its rank in the sub-cell by (sub-bin key, previous index) -- with
-// the sub-cells in key order this is the stable sort of the 2-level Morton
-// key -- gives its sorted slot; every per-particle array moves there in the
-// other buffer, and the sorted slot records its sub-cell (in pslot, free after
-// k_scatter) for the neighbor-row build.
+// K4: one thread per particle, by its previous slot i: its rank in its
+// sub-cell by (sub-bin key, previous index) -- with the sub-cells in key
+// order this is the stable sort of the 2-level Morton key -- gives its sorted
+// slot; every per-particle array moves there in the other buffer (loads in
+// the previous order, issued while the rank is found; stores scattered), and
+// the sorted slot records its sub-cell (in pslot, free after k_scatter) for
+// the neighbor-row build.
 // --------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= P.nact) return;
-    const int s = P.wave_sys[f >> 6];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nact) return;
+    const int s = P.wave_sys[i >> 6];
     if (s < 0) return;
     const SysDev *d = P.sys + s;
     if (!d->active || !d->stale) return;
-    if (f >= d->part_base + d->n) return;
-    const int idx = P.tmp[f];
-    const int key = P.cl[f];
-    const int cell = P.pcell[idx];
+    if (i >= d->part_base + d->n) return;
+    const int cell = P.pcell[i];
+    const int key = P.psub[i];
+    const int cur = d->sel, cura = d->sela;
+    const float4 p = P.pos[cur][i];
+    float4 v = P.vel[cur][i];
+    const int tg = P.tag[cura][i];
+    const int mk = P.mask[cura][i], og = P.origin[cura][i], im = P.image[cura][i];
+    const float ph = P.phi[cura][i];
     const int start = P.cell_start[cell], end = P.cell_start[cell + 1];
     // four members' keys in flight per round trip (loads clamped to the
     // segment, members past its end not counted)
@@ -269,13 +275,9 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            rank += g0 + u < end && ((kg[u] < key) || (kg[u] == key && jg[u] < idx));
+            rank += g0 + u < end && ((kg[u] < key) || (kg[u] == key && jg[u] < i));
     }
     const int dst = start + rank;
-    const int cur = d->sel, cura = d->sela;
-    const float4 p = P.pos[cur][idx];
-    float4 v = P.vel[cur][idx];
-    const int tg = P.tag[cura][idx];
     if (d->phase == PH_FRESH) {
         // a newly loaded system: signatures of its first force pass
         const uint32_t sg = d->style == STYLE_MDPD ? sig_mdpd(tg, d->seed_cur)
@@ -286,12 +288,12 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
     P.vel[cur ^ 1][dst] = v;
     P.xhold[dst] = p;
     P.tag[cura ^ 1][dst] = tg;
-    P.mask[cura ^ 1][dst] = P.mask[cura][idx];
-    P.origin[cura ^ 1][dst] = P.origin[cura][idx];
-    P.image[cura ^ 1][dst] = P.image[cura][idx];
-    P.phi[cura ^ 1][dst] = P.phi[cura][idx];
+    P.mask[cura ^ 1][dst] = mk;
+    P.origin[cura ^ 1][dst] = og;
+    P.image[cura ^ 1][dst] = im;
+    P.phi[cura ^ 1][dst] = ph;
     P.pslot[dst] = cell;
-    if (P.key) P.key[(size_t)P.cap + dst] = P.key[idx];
+    if (P.key) P.key[(size_t)P.cap + dst] = P.key[i];
 }
 
 // --------------------------------------------------------------------------
