@@ -320,7 +320,9 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 // --------------------------------------------------------------------------
 constexpr int NL_ITER = 8;
 
-__global__ void __launch_bounds__(WAVE, 6) k_nlist(PoolDev P, int relist, int ncells)
+// eight waves per SIMD (64 VGPRs, a few values spilled outside the test
+// loop): measured faster than seven waves without spills
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8, 8))) k_nlist(PoolDev P, int relist, int ncells)
 {
     __shared__ int s_row[NL_ITER * WAVE];                   // one centre's entries of one batch
     const int cs = xcd_logical(blockIdx.x, ncells);        // cell slot (Morton rank order)
