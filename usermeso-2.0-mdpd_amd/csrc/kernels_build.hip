@@ -320,8 +320,9 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 // --------------------------------------------------------------------------
 constexpr int NL_ITER = 8;
 
-// eight waves per SIMD (64 VGPRs, a few values spilled outside the test
-// loop): measured faster than seven waves without spills
+// eight waves per SIMD: 64 VGPRs, 48 bytes spilled (reloaded once per
+// candidate batch and once per row store, none in the candidate test);
+// measured faster than seven waves without spills
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8, 8))) k_nlist(PoolDev P, int relist, int ncells)
 {
     __shared__ int s_row[NL_ITER * WAVE];                   // one centre's entries of one batch
