@@ -318,11 +318,10 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 // the rebuild trigger (displacement from xhold) keeps its guarantee.
 // relist selects the systems not flagged stale, otherwise the stale ones.
 // --------------------------------------------------------------------------
-constexpr int NL_ITER = 8;
+constexpr int NL_ITER = 7;             // candidate tiles per batch: 448 candidates (a cell's stencil averages ~370)
 
-// eight waves per SIMD: 64 VGPRs, 48 bytes spilled (reloaded once per
-// candidate batch and once per row store, none in the candidate test);
-// measured faster than seven waves without spills
+// eight waves per SIMD (63 VGPRs with seven tiles in registers, no spills):
+// measured faster than seven waves
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8, 8))) k_nlist(PoolDev P, int relist, int ncells)
 {
     __shared__ int s_row[NL_ITER * WAVE];                   // one centre's entries of one batch
