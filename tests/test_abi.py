@@ -400,3 +400,29 @@ def test_restart_errors(engine, tmp_path):
     with pytest.raises(engine.MdpdError) as e:
         t.command(f"write_restart {tmp_path / 'x.restart'}")
     assert e.value.code == 2
+
+
+def test_system_without_atoms_is_refused_before_the_device(engine):
+    # a complete script with no atoms: setup and pool submission stop at
+    # validation with a state error, before any device work
+    s = engine.System()
+    _deck(s)
+    s.commands("""
+        mass 1 1.0
+        pair_style mdpd/meso 1.0 1.0 5
+        pair_coeff * * -40 25 4.5 1.0 0.75
+        fix int all nve/meso
+        run_style mvv/meso
+    """)
+    assert s.natoms == 0
+    with pytest.raises(engine.MdpdError) as e:
+        s.setup()
+    assert e.value.code == 2 and "no atoms" in str(e.value)
+    with pytest.raises(engine.MdpdError) as e:
+        s.run(5)
+    assert e.value.code == 2
+    p = engine.Pool(64e6, 4)
+    with pytest.raises(engine.MdpdError) as e:
+        p.submit(s, 10)
+    assert e.value.code == 2 and "no atoms" in str(e.value)
+    assert p.stat("queued") == 0
