@@ -8,6 +8,8 @@
   no force, and flies straight (x = x0 + v t).
 * Two particles: the pair force against the oracle.
 * A partial last tile (N = 65: one full tile of 64 and one particle).
+* Particles exactly on the box faces, just below L, and a period or more
+  outside: wrap, sort keys and order bit-exact against the oracle.
 """
 import numpy as np
 import pytest
@@ -77,3 +79,28 @@ def test_two_particles_match_oracle(engine, orc):
     assert np.linalg.norm(f[0]) > 0
     assert_vec_close(s.gather("f"), f, what="two-particle force")
     np.testing.assert_allclose(s.gather("f").sum(axis=0), 0, atol=1e-5)
+
+
+def test_particles_on_the_box_faces(engine, orc):
+    # positions exactly on the faces, the float just below L, tiny negatives
+    # that round onto L after the wrap, and whole periods outside: wrap, keys,
+    # sorted order bit-exact; neighbor sets, densities, forces as above (at a
+    # liquid density: in a sparse box a density made of one weight near r_d
+    # carries the FP32 rounding of r in (r_d - r)^2 at the 1e-5 bound)
+    L, n = 8.0, 2400
+    x, v = uniform_box(n, L, 41)
+    faces = [0.0, L, float(np.nextafter(np.float32(L), np.float32(0))), -1e-7, L + 1e-7, 2 * L, -L, -0.0]
+    for k, val in enumerate(faces):
+        x[k, 0] = val
+        x[20 + k, 1] = val
+        x[40 + k, 2] = val
+    s = mdpd_system(engine, x, v, L)
+    s.setup()
+    origin, keys, xw = s.debug_sorted()
+    xc = orc.wrap(x.astype(np.float32), (0, 0, 0), (L, L, L))
+    assert np.array_equal(xw, xc[origin])
+    ref_keys = orc.sort_keys(xc, (0, 0, 0), (L, L, L), 1.3)
+    assert np.array_equal(keys, ref_keys[origin])
+    assert np.array_equal(origin, orc.stable_sort(ref_keys))
+    f, _ = _check_against_oracle(engine, orc, s, L, n, "faces")
+    assert_vec_close(s.gather("f"), f, what="force with particles on the faces")
