@@ -136,7 +136,9 @@ int mdpd_fix_result(mdpd_system *sys, const char *fix_id, int n, double *x, doub
  * whose launch sizes come from occupancy tables (src/USER-MESO/occupancy_meso.cu).
  * "split": waves sharing one 64-particle tile's neighbor rows in the density
  * and force passes (1, 2 or 4; default 2).  Takes effect at the next pass;
- * results differ from another split only by FP32 summation order. */
+ * results differ from another split only by FP32 summation order.
+ * "graph" 1/0: replay the fused steps of a run from a captured hipGraph
+ * (default 1) or launch their kernels one by one; results are identical. */
 int mdpd_tune(mdpd_system *sys, const char *knob, int value);
 
 /* ---- kernel timing ---------------------------------------------------------

@@ -213,3 +213,28 @@ def test_many_resident_jobs_beyond_one_launch_block(engine):
     got, p = pool_results(engine, [mk(k) for k in range(80)], nsteps, budget=1.0 * GB, max_systems=1024)
     for k in (0, 63, 64, 78, 79):
         assert_same(got[k], plain_result(mk(k), nsteps[k]), f"job {k}")
+
+
+def test_job_finished_before_a_regrow_keeps_its_last_pass(engine):
+    # job a finishes on the device inside a batch of passes whose status word
+    # the host has not read yet (it reads them one batch behind); job b,
+    # admitted next, needs wider neighbor rows, so the pool reallocates them
+    # before a's status is handled.  a's harvest pass (energy and virial of
+    # its last pass) reads that pass's compact rows: a is harvested before
+    # the reallocation, and its results equal a alone.
+    a_sys = lambda: droplet(engine, 2000, 60)
+    alone = pool_results(engine, [a_sys()], [3], poll=8)[0][0]
+    p = engine.Pool(0.5 * GB, 4)
+    p.tune("split", 2)
+    p.tune("poll", 8)
+    ja = p.submit(a_sys(), 3)
+    p.advance(max_passes=8)                         # a takes 4 passes and is done on the device
+    assert p.state(ja) == engine.JOB_RUNNING        # its word is not handled yet
+    jb = p.submit(droplet(engine, 1500, 61, rho_est=15.0), 4)
+    width = p.stat("maxnbr")
+    p.run()
+    assert p.stat("maxnbr") > width and p.stat("regrows") >= 1
+    assert p.state(ja) == engine.JOB_DONE and p.state(jb) == engine.JOB_DONE
+    got = ({k: p.gather(ja, k) for k in ("x", "v", "f", "image", "e_pair", "virial")}, p.thermo(ja),
+           p.info(ja)["timestep"])
+    assert_same(got, alone, "job finished before a regrow")

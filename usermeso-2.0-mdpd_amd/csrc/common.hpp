@@ -117,6 +117,7 @@ struct SysDev {
     // pooled engine: Neighbor::decide on the device (src/neighbor.cpp:1216-1290)
     int ago;                     // steps since the last build
     int n_every, n_delay, n_check;
+    int builds;                  // neighbor builds committed (k_commit_rebuild)
 };
 
 // Device arrays of one pool (a pool packs one or more systems).

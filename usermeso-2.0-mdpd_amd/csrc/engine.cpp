@@ -216,6 +216,9 @@ int mdpd_tune(mdpd_system *sys, const char *knob, int value)
         if (!strcmp(knob, "split")) {
             if (value != 1 && value != 2 && value != 4) fail(MDPD_E_ARG, "split must be 1, 2 or 4");
             sys->split = value;
+            sys->graph_valid = false;
+        } else if (!strcmp(knob, "graph")) {
+            sys->use_graph = value != 0;
         } else {
             fail(MDPD_E_ARG, "unknown tuning knob '%s'", knob);
         }

@@ -14,6 +14,14 @@ enum ForceMode : int {
     MODE_HARVEST = 5,    // pooled engine, finished system: f, e, virial of its last pass
 };
 
+// step commit after a pass (k_commit_step)
+enum CommitKind : int {
+    COMMIT_FINAL = 0,    // end-of-run half-kick: flip the buffer only
+    COMMIT_ADVANCE = 1,  // flip, step + 1, next seed; stale = a particle moved past skin/2
+    COMMIT_PHASED = 2,   // pooled engine: per-system lifecycle + reneighbor decision
+    COMMIT_DECIDE = 3,   // plain engine: COMMIT_ADVANCE with the reneighbor decision
+};
+
 // rebuild phase: wrap, keys, sort, gather, neighbor rows (stale systems only)
 hipError_t launch_rebuild(const PoolDev &P, hipStream_t st);
 hipError_t launch_nlist_only(const PoolDev &P, hipStream_t st);
@@ -27,7 +35,7 @@ hipError_t launch_signatures(const PoolDev &P, hipStream_t st);
 hipError_t launch_first(const PoolDev &P, bool with_rho, int split, hipStream_t st);
 hipError_t launch_initial(const PoolDev &P, hipStream_t st);
 // after launch_initial / a fused force pass (advance) or a final pass (!advance)
-// kind 0 final, 1 advance, 2 phased (pooled); split = waves per tile of the pass
+// kind: CommitKind; split = waves per tile of the pass
 hipError_t launch_commit_step(const PoolDev &P, int kind, int split, hipStream_t st);
 hipError_t launch_force(const PoolDev &P, int style, int mode, bool uni, int split, hipStream_t st);
 // the same over the 64-particle tiles [tile0, tile0 + ntiles) only

@@ -469,6 +469,7 @@ __global__ void k_commit_rebuild(PoolDev P)
     d->sela ^= 1;
     d->stale = 0;
     d->ago = 0;
+    d->builds += 1;
 }
 
 // --------------------------------------------------------------------------

@@ -41,20 +41,41 @@ __device__ inline void tile_flags(int *moved, int tile0, int ntile, int flag)
     if (threadIdx.x < ntile) moved[tile0 + threadIdx.x] = s_or[threadIdx.x];
 }
 
-__device__ inline void apply_image(uint32_t code, const SysDev *d, float &x, float &y, float &z)
+// a - (b + s L) with a single rounding.  Adding s L to the neighbor's
+// coordinate first rounds at the magnitude of L (half an ulp of L: 4.8e-7 at
+// L = 8), an error in r that the density weight (r_d - r)^2 amplifies for a
+// neighbor near r_d.  Here the difference a - b and its exact rounding error
+// (Knuth's TwoSum) come first; a - b lies within r_c + skin of s L, so for
+// L >= 2 (r_c + skin) subtracting s L is exact (Sterbenz) and the error term
+// is added back last, giving the correctly rounded displacement.  With s = 0
+// the result equals a - b bit for bit.
+__device__ inline float image_diff(float a, float b, float sl)
 {
-    int sx, sy, sz;
-    image_decode(code, sx, sy, sz);
-    x += (float)sx * d->L[0];
-    y += (float)sy * d->L[1];
-    z += (float)sz * d->L[2];
+    const float d = a - b;
+    const float bv = d - a;
+    const float e = (a - (d - bv)) + (-b - bv);
+    return (d - sl) + e;
 }
 
-__device__ inline float pair_rsq(const float4 &pi, float4 q, uint32_t code, const SysDev *d)
+// displacement r_i - (r_j + s L) of a neighbor entry with image code `code`
+__device__ inline void pair_delta(const float4 &pi, const float4 &q, uint32_t code, const SysDev *d, float &dx,
+                                  float &dy, float &dz)
 {
-    float xj = q.x, yj = q.y, zj = q.z;
-    if (code != IMG_ZERO) apply_image(code, d, xj, yj, zj);
-    const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+    if (code == IMG_ZERO) {
+        dx = pi.x - q.x; dy = pi.y - q.y; dz = pi.z - q.z;
+    } else {
+        int sx, sy, sz;
+        image_decode(code, sx, sy, sz);
+        dx = image_diff(pi.x, q.x, (float)sx * d->L[0]);
+        dy = image_diff(pi.y, q.y, (float)sy * d->L[1]);
+        dz = image_diff(pi.z, q.z, (float)sz * d->L[2]);
+    }
+}
+
+__device__ inline float pair_rsq(const float4 &pi, const float4 &q, uint32_t code, const SysDev *d)
+{
+    float dx, dy, dz;
+    pair_delta(pi, q, code, d, dx, dy, dz);
     return dx * dx + dy * dy + dz * dz;
 }
 
@@ -274,9 +295,8 @@ __device__ inline void pair_term(Acc<EV> &a, const float4 &pi, const float4 &vi,
                                  const float4 &vj, float rho_j, uint32_t code, const PairC &c, float dtis,
                                  const SysDev *d)
 {
-    float xj = q.x, yj = q.y, zj = q.z;
-    if (code != IMG_ZERO) apply_image(code, d, xj, yj, zj);
-    const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+    float dx, dy, dz;
+    pair_delta(pi, q, code, d, dx, dy, dz);
     const float rsq = dx * dx + dy * dy + dz * dz;
     const bool in = rsq < c.rc2 && rsq >= 1e-20f;
     const float rinv = __builtin_amdgcn_rsqf(rsq);
@@ -336,9 +356,11 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
     SysDev *d = P.sys + s;
     if (!d->active && MODE != MODE_HARVEST) return;
     if (UNI != (d->ntypes == 1)) return;     // host launches the matching variant
-    // a pool may mix styles: each launch takes the systems of its style, and
-    // a system waiting for wider neighbor rows is skipped
-    if (MODE == MODE_PHASED && (d->style != STYLE || d->overflow)) return;
+    // a system waiting for wider neighbor rows is skipped by the stepping
+    // passes (no state change until the host has regrown the rows), and a
+    // pool may mix styles: each launch takes the systems of its style
+    if (STEP && d->overflow) return;
+    if (MODE == MODE_PHASED && d->style != STYLE) return;
     if (MODE == MODE_HARVEST && d->split_last != SPLIT) return;   // the split of its last pass
     const bool valid = i < d->part_base + d->n;
     const int cur = MODE == MODE_HARVEST ? d->sel ^ 1 : d->sel;
@@ -496,8 +518,8 @@ __global__ void __launch_bounds__(WAVE) k_aboundary(PoolDev P, int ntiles, int p
     const int s = P.wave_sys[tile];
     if (s < 0) return;
     const SysDev *d = P.sys + s;
-    if (!d->active || !d->fix_arb) return;
-    if (pooled && (d->overflow || d->phase == PH_LAST)) return;
+    if (!d->active || !d->fix_arb || d->overflow) return;
+    if (pooled && d->phase == PH_LAST) return;
     const int lane = threadIdx.x;
     const int i = tile * WAVE + lane;
     const bool valid = i < d->part_base + d->n;
@@ -518,11 +540,8 @@ __global__ void __launch_bounds__(WAVE) k_aboundary(PoolDev P, int ntiles, int p
                 const uint32_t e = (uint32_t)row[(size_t)(q & ~(NBR_CHUNK - 1)) * WAVE + (q & (NBR_CHUNK - 1))];
                 const uint32_t j = e & IDX_MASK;
                 if (!(mask[j] & d->fix_wall_bit)) continue;
-                const float4 pj = pos[j];
-                float xj = pj.x, yj = pj.y, zj = pj.z;
-                const uint32_t code = e >> IMG_SHIFT;
-                if (code != IMG_ZERO) apply_image(code, d, xj, yj, zj);
-                const float dx = p.x - xj, dy = p.y - yj, dz = p.z - zj;
+                float dx, dy, dz;
+                pair_delta(p, pos[j], e >> IMG_SHIFT, d, dx, dy, dz);
                 const float rsq = dx * dx + dy * dy + dz * dz;
                 if (!(rsq < c2)) continue;
                 const float rinv = __builtin_amdgcn_rsqf(rsq);
@@ -564,20 +583,23 @@ __global__ void __launch_bounds__(WAVE) k_aboundary(PoolDev P, int ntiles, int p
 // Step commit, one workgroup per system, after a pass that wrote the other
 // buffer: ADVANCE (initial kick or fused step) flips the buffer, advances the
 // step, precomputes the next TEA seed and ORs the tiles' moved flags into
-// `stale`; FINAL (end-of-run half-kick) only flips the buffer.  PHASED (pooled
-// engine) follows each system's lifecycle and makes the reneighbor decision
-// of Neighbor::decide (src/neighbor.cpp:1216-1290) on the device: after
-// `ago` steps, rebuild when ago >= delay, ago % every == 0 and (no check, or
-// a particle moved more than skin/2 since the last build).
+// `stale`; FINAL (end-of-run half-kick) only flips the buffer.  DECIDE (plain
+// engine) and PHASED (pooled engine) make the reneighbor decision of
+// Neighbor::decide (src/neighbor.cpp:1216-1290) on the device: after `ago`
+// steps since the last build, rebuild when ago >= delay, ago % every == 0 and
+// (no check, or a particle moved more than skin/2 since the last build);
+// PHASED also follows each system's lifecycle.  A system whose rows
+// overflowed was skipped by the pass and is not advanced.
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance, int split)  // 0 final, 1 advance, 2 phased
+__global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance, int split)
 {
     __shared__ int s_any;
     SysDev *d = P.sys + blockIdx.x;
     if (!d->active) return;
-    if (advance == 2 && d->overflow) {
+    if (d->overflow) {
         // skipped by the passes: no state change until the rows are regrown
-        if (threadIdx.x == 0) P.status[blockIdx.x] = (d->job << 3) | (d->stale ? 1 : 0) | 4;
+        if (advance == COMMIT_PHASED && threadIdx.x == 0)
+            P.status[blockIdx.x] = (d->job << 3) | (d->stale ? 1 : 0) | 4;
         return;
     }
     if (threadIdx.x == 0) s_any = 0;
@@ -590,7 +612,7 @@ __global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance, int
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (advance == 2) {
+        if (advance == COMMIT_PHASED) {
             // pooled engine: per-system lifecycle
             d->split_last = split;
             if (d->phase == PH_LAST) {
@@ -606,7 +628,12 @@ __global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance, int
                 d->phase = d->steps_left <= 0 ? PH_LAST : PH_RUN;
             }
             P.status[blockIdx.x] = (d->job << 3) | (d->stale ? 1 : 0) | (d->phase == PH_DONE ? 2 : 0);
-        } else if (advance) {
+        } else if (advance == COMMIT_DECIDE) {
+            commit_advance(d, false);
+            const int ago = ++d->ago;
+            const bool due = ago >= d->n_delay && ago % d->n_every == 0;
+            d->stale = due && (!d->n_check || s_any != 0) ? 1 : 0;
+        } else if (advance == COMMIT_ADVANCE) {
             commit_advance(d, s_any != 0);
         } else {
             d->sel ^= 1;

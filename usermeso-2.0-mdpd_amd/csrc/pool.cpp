@@ -404,11 +404,36 @@ struct mdpd_pool {
         graph_valid = false;
     }
 
+    // Systems that finished on the device but whose status word the host has
+    // not handled yet (it reads them one batch behind) are harvested now,
+    // from the device descriptors: their harvest pass (energy and virial of
+    // the last pass) reads the compact rows of that pass, which a regrow
+    // reallocates.  The stream is drained on return, so the harvest pass has
+    // run before any row array is freed.
+    void harvest_finished_now()
+    {
+        std::vector<SysDev> h(max_sys);
+        hip_check(hipMemcpyAsync(h.data(), dsys.p, sizeof(SysDev) * max_sys, hipMemcpyDeviceToHost, stream),
+                  "readback");
+        hip_check(hipStreamSynchronize(stream), "readback");
+        bool any = false;
+        for (int s = 0; s < max_sys; ++s) {
+            const int jid = slot_job[s];
+            if (!jid || h[s].job != jid || h[s].phase != PH_DONE) continue;
+            Job &j = *jobs[jid];
+            if (j.harvesting) continue;
+            start_harvest(j);
+            any = true;
+        }
+        if (any) hip_check(hipStreamSynchronize(stream), "harvest before regrow");
+    }
+
     // wider neighbor rows for every system, rows rebuilt in place (no re-sort,
     // so results do not change); the stream is drained first
     void regrow(int want)
     {
         hip_check(hipStreamSynchronize(stream), "regrow");
+        harvest_finished_now();
         // status words read before the regrow are stale (overflow flags are
         // cleared below; finished systems keep their word and show up again)
         poll_pending[0] = poll_pending[1] = false;
@@ -647,7 +672,7 @@ struct mdpd_pool {
         for (auto &c : combos) hip_check(launch_force(P, c.first, MODE_PHASED, c.second != 0, sp, stream), "force");
         if (with_arb) hip_check(launch_aboundary(P, 1, stream), "aboundary");
         mark(PP_COMMIT);
-        hip_check(launch_commit_step(P, 2, sp, stream), "commit step");
+        hip_check(launch_commit_step(P, COMMIT_PHASED, sp, stream), "commit step");
         mark(PP_COUNT);
     }
 

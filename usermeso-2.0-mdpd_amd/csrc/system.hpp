@@ -234,8 +234,17 @@ struct mdpd_system {
     double prof_ms[PH_COUNT] = {0};
     long long prof_n[PH_COUNT] = {0};
 
+    // ---- step replay: a fused step (rebuild phase, passes, commit with the
+    // reneighbor decision) captured once as a hipGraph and replayed; the host
+    // reads the device step counter once per batch of STEP_BATCH steps ----
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_valid = false;
+    static constexpr int STEP_BATCH = 64;
+
     ~mdpd_system()
     {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto &f : fixes) {
             try { finish_obs(f); } catch (...) {}
         }
@@ -1604,8 +1613,10 @@ struct mdpd_system {
         hip_check(hipMemsetAsync(frc.p, 0, cap * sizeof(float4), stream), "memset");
         hip_check(hipMemsetAsync(vend.p, 0, cap * sizeof(float4), stream), "memset");
         hip_check(hipMemsetAsync(vir.p, 0, 6 * cap * sizeof(float), stream), "memset");
+        d.builds = (int)nbuild;       // the device counts builds from here on
         hsys = d;
         up(dsys.p, &hsys, sizeof(SysDev));
+        graph_valid = false;
 
         rebuild();
         PoolDev P = view();
@@ -1613,6 +1624,7 @@ struct mdpd_system {
         force_pass(MODE_SETUP);
         hip_check(hipStreamSynchronize(stream), "setup");
         collect_events();
+        nbuild = read_field(offsetof(SysDev, builds));
         setup_done = true;
         dirty = false;
     }
@@ -1639,7 +1651,28 @@ struct mdpd_system {
             hip_check(launch_nlist_only(P, stream), "nlist");
         }
         hip_check(launch_commit_rebuild(P, stream), "commit");
-        ++nbuild;
+        graph_valid = false;
+    }
+
+    // Rows of a step's rebuild overflowed on the device (that step's passes
+    // and commit skipped the system): widen the rows and re-list the sorted
+    // system in place from the positions of its last build, which clears
+    // the flag; the skipped step then runs again.  Stream drained on entry.
+    void regrow_rows()
+    {
+        for (int attempt = 0;; ++attempt) {
+            if (attempt > 4) fail(MDPD_E_OVERFLOW, "<MESO> Pair table overflow; local density too high");
+            maxnbr = (int)std::ceil(maxnbr * 1.5 / NBR_ALIGN) * NBR_ALIGN;
+            nlist.alloc((size_t)cap * maxnbr);
+            clist.alloc((size_t)cap * maxnbr);
+            hsys.maxnbr = maxnbr;
+            hip_check(hipMemcpyAsync((char *)dsys.p + offsetof(SysDev, maxnbr), &maxnbr, sizeof(int),
+                                     hipMemcpyHostToDevice, stream), "maxnbr");
+            graph_valid = false;
+            PoolDev P = view();
+            hip_check(launch_relist(P, stream), "relist");
+            if (!read_field(offsetof(SysDev, overflow))) return;
+        }
     }
 
     hipEvent_t take_event()
@@ -1692,17 +1725,66 @@ struct mdpd_system {
         if (t >= 0 && !obs.empty()) obs_post(t);
         if ((mode == MODE_FUSED || mode == MODE_FUSED_OUT) && hsys.fix_arb)
             hip_check(launch_aboundary(P, 0, stream), "aboundary");          // post_integrate
-        if (mode != MODE_SETUP) hip_check(launch_commit_step(P, mode != MODE_FINAL, split, stream), "commit");
+        if (mode != MODE_SETUP)
+            hip_check(launch_commit_step(P, mode == MODE_FINAL ? COMMIT_FINAL : COMMIT_DECIDE, split, stream), "commit");
     }
 
-    // LAMMPS Neighbor::decide with the flag computed on the device
-    bool want_rebuild(long long ago, int stale)
+    // The rebuild phase of a step, launched every step: its kernels re-sort
+    // and re-list the system only when the previous commit flagged it stale
+    // (Neighbor::decide made on the device), otherwise they exit at once.
+    void rebuild_phase()
     {
-        if (!(ago >= delay && ago % every == 0)) return false;
-        if (!check) return true;
-        return stale != 0;
+        PoolDev P = view();
+        timed(PH_REBUILD, [&]() { hip_check(launch_rebuild(P, stream), "rebuild"); });
+        hip_check(launch_commit_rebuild(P, stream), "commit rebuild");
     }
 
+    // one fused step (rebuild phase, passes, commit) as a hipGraph
+    void capture_step()
+    {
+        if (gexec) {
+            hip_check(hipGraphExecDestroy(gexec), "hipGraphExecDestroy");
+            gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            rebuild_phase();
+            force_pass(MODE_FUSED);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "hipGraphInstantiate");
+        graph_valid = true;
+    }
+
+    // Drain the stream and read the device step counter; rows that
+    // overflowed in a rebuild are widened (the step that found them was
+    // skipped and runs again).  Returns the step of the next force pass.
+    long long settle()
+    {
+        hip_check(hipMemcpyAsync(pinned, (char *)dsys.p + offsetof(SysDev, step), sizeof(long long),
+                                 hipMemcpyDeviceToHost, stream), "readback");
+        hip_check(hipMemcpyAsync(pinned + 2, (char *)dsys.p + offsetof(SysDev, overflow), sizeof(int),
+                                 hipMemcpyDeviceToHost, stream), "readback");
+        hip_check(hipStreamSynchronize(stream), "readback sync");
+        long long step;
+        memcpy(&step, pinned, sizeof(step));
+        if (pinned[2]) regrow_rows();
+        return step;
+    }
+
+    // run N (ModifiedVerlet::run, mvv_meso.cu:309-572).  The reneighbor
+    // decision (Neighbor::decide, src/neighbor.cpp:1216-1290) is made by each
+    // step's commit on the device, counting steps since the last build across
+    // runs; the host does not wait on the device inside the step loop except
+    // once per batch of replayed steps, and at steps with host work (dumps,
+    // observables, timed phases, the final half-kick).
     void run(long long nsteps)
     {
         if (nsteps < 0) fail(MDPD_E_ARG, "invalid run length");
@@ -1713,26 +1795,37 @@ struct mdpd_system {
         PoolDev P = view();
         timed(PH_INITIAL, [&]() { hip_check(launch_initial(P, stream), "initial"); });
         if (hsys.fix_arb) hip_check(launch_aboundary(P, 0, stream), "aboundary");   // post_integrate
-        hip_check(launch_commit_step(P, 1, split, stream), "commit");
-        long long ago = 0;
-        for (long long k = 1; k <= nsteps; ++k) {
-            ++ago;
-            int stale = read_field(offsetof(SysDev, stale));
-            if (want_rebuild(ago, stale)) {
-                if (!stale) {   // forced by every/delay without check
-                    int one = 1;
-                    hip_check(hipMemcpyAsync((char *)dsys.p + offsetof(SysDev, stale), &one, sizeof(int),
-                                             hipMemcpyHostToDevice, stream), "stale");
-                }
-                rebuild();
-                ago = 0;
+        hip_check(launch_commit_step(P, COMMIT_DECIDE, split, stream), "commit");
+        const long long t0 = ntimestep;
+        long long k = 1;                  // the step whose force pass runs next
+        while (k < nsteps) {
+            const long long t = t0 + k;
+            const bool out = !dumps.empty() && dump_due(t);
+            if (out || !obs.empty() || prof || !use_graph) {
+                // a step with host work: its rebuild settles before the passes
+                rebuild_phase();
+                if (settle() != t) fail(MDPD_E_HIP, "device step counter out of order at step %lld", t);
+                force_pass(out ? MODE_FUSED_OUT : MODE_FUSED, t);
+                if (out) write_dumps(t, true);
+                ++k;
+                continue;
             }
-            const bool out = !dumps.empty() && dump_due(ntimestep + k);
-            force_pass(k < nsteps ? (out ? MODE_FUSED_OUT : MODE_FUSED) : MODE_FINAL, ntimestep + k);
-            if (out) write_dumps(ntimestep + k, k < nsteps);
+            long long m = 1;
+            while (m < STEP_BATCH && k + m < nsteps && !(!dumps.empty() && dump_due(t + m))) ++m;
+            if (!graph_valid) capture_step();
+            for (long long q = 0; q < m; ++q) hip_check(hipGraphLaunch(gexec, stream), "hipGraphLaunch");
+            k = settle() - t0;
+            if (k < 1 || k > nsteps) fail(MDPD_E_HIP, "device step counter out of range (%lld)", k + t0);
         }
+        // the final half-kick: its rebuild settles first, so the pass runs
+        rebuild_phase();
+        if (settle() != t0 + nsteps) fail(MDPD_E_HIP, "device step counter out of order at the last step");
+        const bool out = !dumps.empty() && dump_due(t0 + nsteps);
+        force_pass(MODE_FINAL, t0 + nsteps);
+        if (out) write_dumps(t0 + nsteps, false);
         hip_check(hipStreamSynchronize(stream), "run");
         collect_events();
+        nbuild = read_field(offsetof(SysDev, builds));
         ntimestep += nsteps;
         host_stale = true;
     }
