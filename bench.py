@@ -1,32 +1,41 @@
 """mDPD throughput benchmark on one MI355X: batched / served vs plain.
 
-Default workload ("batched"): 256 independent periodic boxes, each a
-spherical mDPD droplet of N particles, N ~ LogNormal(ln 8000, 0.5) clipped to
-[2000, 40000], uniform in a sphere of radius R = (3N / (4 pi 6.05))^(1/3) at
-density 6.05 in a cube of side 2.5 R; Maxwell-Boltzmann velocities at kT = 1;
-A = -40, B = 25, r_c = 1, r_d = 0.75, gamma = 4.5, dt = 0.01, quadratic
-density weight, skin 0.3, neigh_modify every 1 delay 0 check yes; `--steps`
-(2000) steps per job.  All jobs are admitted to one pool (inputs resident in
-HBM) before the timed region; the timed region is the whole batched run
-(autotuned split, hipGraph-replayed passes, harvest of every job's results).
-`value` is total particle-steps / wall time.  The plain path (one system per
-handle, one launch sequence per system and step) runs a stratified sample of
-the same jobs for the same number of steps, timed per job after its setup.
+Default run (one JSON line; progress on stderr):
 
-The served leg (`--served-seconds`, 0 to skip) submits jobs with Poisson
-arrivals (`--served-rate` per second): each a droplet or a slab (50/50), N
-log-uniform in [1000, 200000], `--steps` steps, admitted by the pool's HBM
-budget while others run; it reports throughput and p50/p99 latency from
-arrival to result.  The jobs' synthetic particles are made before the clock
-starts (client data); building each system and submitting it is timed.
+* batched (`value`): 256 independent periodic boxes, each a spherical mDPD
+  droplet of N particles, N ~ LogNormal(ln 8000, 0.5) clipped to [2000, 40000],
+  uniform in a sphere of radius R = (3N / (4 pi 6.05))^(1/3) at density 6.05 in
+  a cube of side 2.5 R; Maxwell-Boltzmann velocities at kT = 1; A = -40,
+  B = 25, r_c = 1, r_d = 0.75, gamma = 4.5, dt = 0.01, quadratic density
+  weight, skin 0.3, neigh_modify every 1 delay 0 check yes; 2000 steps per
+  job (`--job-steps`).  All jobs are admitted to one pool (inputs resident in
+  HBM) before the clock starts; the timed region is the whole batched run
+  (autotuned split, hipGraph-replayed passes, harvest of every job's
+  results); `value` = total particle-steps / its wall time.  The plain path
+  (one system per handle, its fused steps replayed from a hipGraph with the
+  reneighbor decision on the device) runs a size-stratified sample of the
+  same jobs for the same 2000 steps, timed per job after its setup.
+* served: Poisson arrivals at 50 jobs/s for 120 s, each job a droplet or a
+  slab (50/50) with N log-uniform in [1000, 200000], 2000 steps, admitted by
+  the pool's HBM budget while others run; throughput and p50/p99 latency from
+  arrival to result; a size-stratified sample of the same jobs on the plain
+  path.
+* slab: the plain liquid slab (box 40 x 40 x 20 with N = 96,800, and
+  40 x 80 x 20 with N = 193,600; slab z in [5, 15) at density 6.05, Lucy
+  weight), 5000 steps each, with the force-kernel and whole-step fractions of
+  HBM peak from a separate profiled run.
+* CPU baselines (GPU box host): the FP64 transcription of LAMMPS pair_style
+  dpd + fix nve on the classic DPD fluid (L = 22, N = 32,000, a = 25,
+  gamma = 4.5, dt = 0.04), on all physical cores and on one thread, and the
+  FP64 mDPD oracle on the median droplet, each over a bounded sample.
 
-Other workloads on the plain path (one handle): `slab` (box 40 x 80 x 20,
-slab z in [5, 15) at density 6.05, N = 193,600, Lucy weight; `slab-literal`:
-40 x 40 x 20, N = 96,800), `pore` (quasi-2D pore flow, 200 x 100 x 4, frozen
-disc walls, body force; ~480,000 particles) and `dpd` (classic DPD fluid,
-L = 22, N = 32,000, beside the CPU LAMMPS pair_style dpd transcription).
+`--steps K` and `--warmup W` keep the benchmark contract: W untimed passes
+on a warm-up pool, and `ms_per_step` = the timed region / K; the jobs run
+2000 steps whatever K is.
 
-Prints one JSON line.
+Other workloads (`--workload`): batched, served, slab, slab-literal, pore
+(quasi-2D pore flow, 200 x 100 x 4, frozen disc walls, body force; ~480,000
+particles), dpd (the classic DPD fluid on the GPU beside its CPU baseline).
 """
 import argparse
 import json
@@ -46,10 +55,13 @@ NLIST = RHO * 4.0 / 3.0 * np.pi * (RC + SKIN) ** 3          # 55.7 entries per r
 B_FORCE = 12 + 12 + 4 + 4 + 4 + 12 + 12 + 4 * NLIST          # bytes per particle, force + fused VV pass
 B_DENSITY = 12 + 4 + 4 * NLIST                               # bytes per particle, density pass
 B_STEP = B_FORCE + B_DENSITY                                 # 522 B per particle-step
+# compulsory bytes of one neighbor build per particle: read the state
+# (x, v, tag, mask, origin, image, phi: 52 B), write it sorted (52 B) and the
+# build positions (16 B), write the row (4 B per entry) and its count
+B_REBUILD = 52 + 52 + 16 + 4 * NLIST + 4                     # 347 B per particle-build
 # HBM bytes per particle moved by the pooled force pass, measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (medians over the passes of 64
-# droplets, counters scaled by a 1 GiB copy: x2.0 fetch, x1.0 write):
-# 181.4 read + 32.1 written
+# droplets, counters scaled by a 1 GiB copy: x2.0 fetch, x1.0 write)
 T_FORCE_POOL = 181.4 + 32.1
 GB = float(1 << 30)
 
@@ -69,6 +81,12 @@ DECK = """
     timestep 0.01
     run_style mvv/meso
 """
+
+T_START = time.perf_counter()
+
+
+def log(msg):
+    print(f"[bench {time.perf_counter() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def _mb(rng, n):
@@ -167,25 +185,49 @@ def system(box, x, v, seed, lucy=False):
 # ---------------------------------------------------------------------------
 # plain path: one handle per system
 # ---------------------------------------------------------------------------
-def plain_job(box, x, v, steps, seed, lucy=False, split=None, profile=False):
+def plain_job(box, x, v, steps, seed, lucy=False, split=None, graph=True):
+    """Wall time of `run steps` after setup (the run returns after the
+    stream has drained)."""
     s = system(box, x, v, seed, lucy)
     if split:
         s.tune("split", split)
+    if not graph:
+        s.tune("graph", 0)
     s.setup()
-    if profile:
-        s.profile(True)
     t0 = time.perf_counter()
-    s.run(steps)                      # returns after the stream has drained
+    s.run(steps)
     dt = time.perf_counter() - t0
-    out = {"dt": dt, "nbuild": s.nbuild}
-    if profile:
-        out["prof"] = {ph: s.profile_get(ph) for ph in ("force", "density", "rebuild", "initial")}
-        out["thermo"] = s.thermo()
+    out = {"dt": dt, "nbuild": s.nbuild, "temp": s.thermo()["temp"]}
     s.close()
     return out
 
 
-def plain_sample(sizes, spec, steps, k):
+def plain_phases(box, x, v, steps, seed, lucy=False, split=None):
+    """Per-phase kernel times (HIP events around every launch) of a separate
+    run: a profiled run launches each step's kernels one by one."""
+    s = system(box, x, v, seed, lucy)
+    if split:
+        s.tune("split", split)
+    s.setup()
+    s.profile(True)
+    s.run(steps)
+    prof = {ph: s.profile_get(ph) for ph in ("force", "density", "rebuild", "initial")}
+    s.close()
+    return prof
+
+
+def plain_best_split(box, x, v, seed, steps=200, lucy=False):
+    """Launch shape of the plain path for one system: the fastest of 1, 2, 4
+    waves per tile over a short run."""
+    best, times = None, {}
+    for sp in (1, 2, 4):
+        times[sp] = plain_job(box, x, v, steps, seed, lucy=lucy, split=sp)["dt"]
+        if best is None or times[sp] < times[best]:
+            best = sp
+    return best, times
+
+
+def plain_sample(sizes, spec, steps, k, split):
     """Plain runs of k jobs spread evenly over the size-sorted job list;
     spec(i) -> (box, x, v, seed) of job i."""
     order = np.argsort(sizes, kind="stable")
@@ -193,7 +235,7 @@ def plain_sample(sizes, spec, steps, k):
     work = secs = 0.0
     for i in pick:
         box, x, v, seed = spec(int(i))
-        r = plain_job(box, x, v, steps, seed)
+        r = plain_job(box, x, v, steps, seed, split=split)
         work += len(x) * steps
         secs += r["dt"]
     return work / secs, [int(sizes[i]) for i in pick], secs
@@ -202,9 +244,15 @@ def plain_sample(sizes, spec, steps, k):
 # ---------------------------------------------------------------------------
 # batched: one pool, all jobs resident
 # ---------------------------------------------------------------------------
-def batched(specs, steps, warmup, split, budget, graph=True):
+POOL_STATS = ("passes", "split", "capacity", "maxnbr", "bytes", "regrows", "prof_passes", "prof_particles",
+              "prof_rebuild_ms", "prof_first_ms", "prof_force_ms", "prof_commit_ms", "tune_ms_1", "tune_ms_2",
+              "tune_ms_4", "particle_steps", "particle_builds", "captures")
+
+
+def batched(specs, job_steps, warmup, split, budget, graph=True):
     import mdpd355
-    # warm-up: a small pool of the first jobs (loads every kernel, builds a graph)
+    # warm-up: a small pool of the first jobs for `warmup` passes (loads
+    # every kernel, builds a graph); not timed
     if warmup > 0:
         w = mdpd355.Pool(budget, 16)
         if not graph:
@@ -223,25 +271,67 @@ def batched(specs, steps, warmup, split, budget, graph=True):
     ids = []
     for box, x, v, seed in specs:
         s = system(box, x, v, seed)
-        ids.append(p.submit(s, steps))
+        ids.append(p.submit(s, job_steps))
         s.close()
     admitted = p.admit()
     assert admitted == len(specs), f"only {admitted} of {len(specs)} jobs fit the pool"
+    log(f"batched: {len(specs)} jobs resident, running {job_steps} steps each")
     t0 = time.perf_counter()
     p.run()
     dt = time.perf_counter() - t0
-    st = {k: p.stat(k) for k in ("passes", "split", "capacity", "maxnbr", "bytes", "regrows", "prof_passes",
-                                 "prof_rebuild_ms", "prof_first_ms", "prof_force_ms", "prof_commit_ms",
-                                 "tune_ms_1", "tune_ms_2", "tune_ms_4")}
+    st = {k: p.stat(k) for k in POOL_STATS}
     temps = [p.thermo(j)["temp"] for j in ids[:: max(1, len(ids) // 16)]]
     p.close()
     return dt, st, temps
 
 
+def sampled(st):
+    npp = max(st["prof_passes"], 1.0)
+    return {k: st[f"prof_{k}_ms"] / npp for k in ("rebuild", "first", "force", "commit")}
+
+
+def rebuild_record(st, wall_s):
+    """The rebuild phase apart: its sampled share of a pass, how many
+    particle-builds the jobs made per particle-step, and its algorithmic rate."""
+    ms = sampled(st)
+    per_pass = sum(ms.values())
+    t_rebuild = ms["rebuild"] * 1e-3 * st["passes"]
+    bytes_ = st["particle_builds"] * B_REBUILD
+    ach = bytes_ / t_rebuild / 1e9 if t_rebuild > 0 else None
+    return {
+        "ms_per_pass": ms["rebuild"], "share_of_pass_kernel_time": ms["rebuild"] / per_pass if per_pass else None,
+        "particle_builds_per_particle_step": st["particle_builds"] / max(st["particle_steps"], 1.0),
+        "bytes_per_particle_build": B_REBUILD,
+        "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS if ach else None,
+        "note": "rebuild kernels run every pass and exit for systems that are not stale; their time over all "
+                "passes against the compulsory bytes of the builds the jobs made",
+    }
+
+
+def force_roofline(st, kernel, traffic_per_particle=None):
+    """k_force of the pool pass: algorithmic bytes of the particles the sampled
+    passes evaluated over the force phase's event time in those passes."""
+    t = st["prof_force_ms"] * 1e-3
+    parts = st["prof_particles"]
+    if t <= 0 or parts <= 0:
+        return None
+    achieved = B_FORCE * parts / t / 1e9
+    frac = achieved / HBM_PEAK_GBS
+    if frac > 1.0:
+        raise RuntimeError(f"force roofline above peak ({achieved:.0f} GB/s): the sampled passes are not "
+                           f"what their particle count says")
+    per = parts / max(st["prof_passes"], 1.0)
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": frac, "traffic": None if traffic_per_particle is None else traffic_per_particle * per,
+            "bytes_per_particle": B_FORCE, "particles_per_launch": per,
+            "avg_launch_us": st["prof_force_ms"] / max(st["prof_passes"], 1.0) * 1e3,
+            "sampled_passes": st["prof_passes"]}
+
+
 # ---------------------------------------------------------------------------
 # served: Poisson arrivals into a pool with an HBM budget
 # ---------------------------------------------------------------------------
-def served(seconds, rate, steps, budget, seed, max_wall=240.0, split=0, graph=True):
+def served(seconds, rate, steps, budget, seed, max_wall, split=0, graph=True):
     import mdpd355
     rng = np.random.default_rng(seed)
     arr = []
@@ -251,11 +341,19 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0, split=0, graph=Tr
         t += rng.exponential(1.0 / rate)
     kinds = rng.random(len(arr)) < 0.5
     sizes = np.round(np.exp(rng.uniform(np.log(1000.0), np.log(200000.0), len(arr)))).astype(int)
-    spec = lambda i: (make_droplet(sizes[i], 7000 + i) if kinds[i] else make_slab_job(sizes[i], 7000 + i))
-    # the clients' inputs exist before they arrive: the synthetic particles
-    # are made before the clock starts; building the system and submitting
-    # it (the engine's API) is timed
-    inputs = [spec(i) for i in range(len(arr))]
+
+    def spec(i):
+        box, x, v = make_droplet(sizes[i], 7000 + i) if kinds[i] else make_slab_job(sizes[i], 7000 + i)
+        return box, x.astype(np.float32), v.astype(np.float32)
+    # the clients make their synthetic particles ahead of their arrival
+    # times (four producer threads, in arrival order); building the system
+    # and submitting it (the engine's API) is timed
+    from concurrent.futures import ThreadPoolExecutor
+    log(f"served: {len(arr)} jobs ({int(sizes.sum())} particles) made by 4 client threads")
+    producers = ThreadPoolExecutor(4)
+    inputs = [producers.submit(spec, i) for i in range(len(arr))]
+    if arr:
+        inputs[min(len(arr), 64) - 1].result()          # a head start before the clock
     p = mdpd355.Pool(budget, 1024)
     p.tune("poll", 4)
     p.tune("profile", 16)
@@ -265,14 +363,17 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0, split=0, graph=Tr
     meta, lat, work = {}, [], 0.0
     occ = []                                   # (used slot fraction, running jobs, queued jobs) per loop
     t_adv = 0.0                                # wall time inside advance()
+    log(f"served: {seconds:g} s of arrivals at {rate:g}/s, {steps} steps per job")
     t0 = mdpd355.clock()
+    next_log = 15.0
+    truncated = False
     i = 0
     while True:
         now = mdpd355.clock() - t0
         while i < len(arr) and arr[i] <= now:
-            box, x, v = inputs[i]
+            box, x, v = inputs[i].result()
             inputs[i] = None
-            s = system(box, x, v, 9000 + i)
+            s = system(box, x.astype(np.float64), v.astype(np.float64), 9000 + i)
             meta[p.submit(s, steps)] = (t0 + arr[i], len(x))
             s.close()
             i += 1
@@ -291,18 +392,25 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0, split=0, graph=Tr
             p.release(j)
         if i >= len(arr) and not meta:
             break
-        if mdpd355.clock() - t0 > max_wall:
+        if now > next_log:
+            log(f"served: t={now:.0f}s arrived {i}/{len(arr)} done {len(lat)} running {p.stat('running'):.0f} "
+                f"queued {p.stat('queued'):.0f}")
+            next_log += 15.0
+        if now > max_wall:
+            truncated = True
+            log(f"served: stopping at the {max_wall:g} s wall limit with {len(meta)} jobs unfinished")
             break
     wall = mdpd355.clock() - t0
-    st = {k: p.stat(k) for k in ("passes", "particle_steps", "captures", "split", "prof_passes", "prof_rebuild_ms",
-                                 "prof_first_ms", "prof_force_ms", "prof_commit_ms")}
+    for f in inputs[i:]:
+        f.cancel()
+    producers.shutdown(wait=True)
+    st = {k: p.stat(k) for k in POOL_STATS}
     p.close()
     lat = np.array(lat)
-    npp = max(st["prof_passes"], 1.0)
     out = {
-        "arrivals": len(arr), "completed": int(len(lat)), "window_s": seconds, "rate_per_s": rate,
-        "steps_per_job": steps, "budget_GB": budget / GB, "wall_s": wall,
-        "value": work / wall if wall > 0 else 0.0,
+        "arrivals": len(arr), "completed": int(len(lat)), "truncated_at_wall_limit": truncated,
+        "window_s": seconds, "rate_per_s": rate, "steps_per_job": steps, "budget_GB": budget / GB,
+        "wall_s": wall, "value": work / wall if wall > 0 else 0.0,
         "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
         "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
         "offered_particle_steps_per_s": float(sizes.sum()) * steps / seconds if len(arr) else 0.0,
@@ -313,18 +421,80 @@ def served(seconds, rate, steps, budget, seed, max_wall=240.0, split=0, graph=Tr
         "passes": st["passes"], "ms_per_pass": wall * 1e3 / max(st["passes"], 1.0),
         "mean_particles_per_pass": st["particle_steps"] / max(st["passes"], 1.0),
         "graph_captures": st["captures"], "split": st["split"],
-        "sampled_ms_per_pass": {k: st[f"prof_{k}_ms"] / npp for k in ("rebuild", "first", "force", "commit")},
+        "sampled_ms_per_pass": sampled(st),
+        "rebuild": rebuild_record(st, wall),
     }
-    return out, sizes, (lambda i: (*spec(i), 9000 + i))
+    roof = force_roofline(st, "k_force<mDPD, MODE_PHASED> (served pool pass)")
+    if roof:
+        out["force_kernel"] = {k: roof[k] for k in ("achieved", "frac", "avg_launch_us", "particles_per_launch")}
+    def spec64(i):
+        box, x, v = spec(i)
+        return box, x.astype(np.float64), v.astype(np.float64), 9000 + i
+    return out, sizes, spec64
 
 
 # ---------------------------------------------------------------------------
-# CPU baseline: the oracle's FP64 OpenMP step on one job, bounded sample
+# CPU baselines (host of the GPU box)
 # ---------------------------------------------------------------------------
-def cpu_baseline(box, x, v, lucy, label, target_s=12.0, seed=4242):
+def host_cpus():
+    """CPUs this process may run on, their physical cores, model, SMT."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            cores.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    smt = None
+    try:
+        smt = open("/sys/devices/system/cpu/smt/active").read().strip() == "1"
+    except OSError:
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    phys = len(cores)
+    if omp and omp.isdigit():
+        phys = min(phys, int(omp))      # the box's CPU share for this process
+    return {"nproc": len(cpus), "physical_cores": phys, "model": model, "smt_active": smt,
+            "omp_num_threads_env": omp}
+
+
+def cpu_c1(seconds, threads):
+    """FP64 LAMMPS pair_style dpd + fix nve transcription on the classic DPD
+    fluid (config 1), `threads` OpenMP threads, a bounded number of steps."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import orc
-    threads = min(16, os.cpu_count() or 1)
+    L, n = 22.0, 32000
+    rng = np.random.default_rng(20241203)
+    x = _f32(rng.uniform(0, L, (n, 3)))
+    v = _f32(_mb(rng, n))
+    kw = dict(dt=0.04, seed=4321, nthreads=threads)
+    t0 = time.perf_counter()
+    orc.lammps_dpd_run(x, v, (0, 0, 0), (L, L, L), nsteps=3, **kw)
+    per = (time.perf_counter() - t0) / 3
+    steps = int(max(3, min(1000, seconds / per)))
+    t0 = time.perf_counter()
+    out = orc.lammps_dpd_run(x, v, (0, 0, 0), (L, L, L), nsteps=steps, **kw)
+    t = time.perf_counter() - t0
+    return {"value": n * steps / t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 transcription of LAMMPS pair_style dpd + fix nve (half list, newton on, Marsaglia "
+                      f"streams) on the config-1 DPD fluid L=22 N={n} a=25 gamma=4.5 dt=0.04, {steps} of its "
+                      f"1000 steps from the random start incl. neighbor builds, {t:.1f} s wall, {threads} "
+                      f"thread(s)", "builds": out["nbuild"]}
+
+
+def cpu_mdpd(box, x, v, lucy, label, seconds, threads, seed=4242):
+    """The FP64 mDPD oracle step (OpenMP) on one of the benchmark's systems."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import orc
     p = orc.params((0, 0, 0), box, 0.01, cut=RC, cut_r=RD, A=-40.0, B=25.0, gamma=4.5,
                    sigma=np.sqrt(9.0), expw=1.0)
     n = len(x)
@@ -333,25 +503,60 @@ def cpu_baseline(box, x, v, lucy, label, target_s=12.0, seed=4242):
     t0 = time.perf_counter()
     orc.run(x, v, *args, 0, **kw)                         # setup only
     t_setup = time.perf_counter() - t0
-    steps = 2
     t0 = time.perf_counter()
-    orc.run(x, v, *args, steps, **kw)
-    per = max((time.perf_counter() - t0 - t_setup) / steps, 1e-6)
-    steps = int(max(2, min(6000, target_s / per)))
+    orc.run(x, v, *args, 2, **kw)
+    per = max((time.perf_counter() - t0 - t_setup) / 2, 1e-6)
+    steps = int(max(2, min(6000, seconds / per)))
     t0 = time.perf_counter()
     orc.run(x, v, *args, steps, **kw)
     t = time.perf_counter() - t0 - t_setup
     return {"value": n * steps / t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP64 mDPD step (OpenMP, {threads} threads) on {label} N={n}, {steps} steps "
-                      f"after setup, {t:.1f} s wall (setup {t_setup:.1f} s excluded)"}
+            "sample": f"FP64 mDPD oracle step (OpenMP, {threads} threads) on {label} N={n}, {steps} steps after "
+                      f"setup, {t:.1f} s wall (setup {t_setup:.1f} s excluded); the reference has no CPU mDPD"}
 
 
-def roofline(t_force_s, particles, kernel, traffic_per_particle=None):
-    achieved = B_FORCE * particles / t_force_s / 1e9
-    traffic = None if traffic_per_particle is None else traffic_per_particle * particles
-    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_particle": B_FORCE,
-            "particles_per_launch": particles, "avg_launch_us": t_force_s * 1e6}
+def cpu_baselines(a, droplet=None):
+    hw = host_cpus()
+    cores = max(1, hw["physical_cores"])
+    log(f"cpu: config-1 DPD on {cores} cores and 1 thread ({hw['model']}, nproc {hw['nproc']})")
+    allc = cpu_c1(a.cpu_seconds, cores)
+    one = cpu_c1(a.cpu_seconds, 1)
+    out = dict(allc)
+    out["host"] = hw
+    out["one_thread"] = {k: one[k] for k in ("value", "cores", "sample")}
+    if droplet is not None:
+        box, x, v, label = droplet
+        log("cpu: mDPD oracle on the median droplet")
+        out["mdpd_oracle"] = cpu_mdpd(box, x, v, False, label, a.cpu_seconds, cores)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# slab records (plain path)
+# ---------------------------------------------------------------------------
+def slab_record(name, steps, split=None, profile_steps=200):
+    w = SLABS[name]
+    box = w["box"]
+    x, v = make_slab(box, w["slab"])
+    n = len(x)
+    if split is None:
+        split, _ = plain_best_split(box, x, v, 4242, steps=100, lucy=True)
+    plain_job(box, x, v, 20, 4242, lucy=True, split=split)      # warm-up
+    r = plain_job(box, x, v, steps, 4242, lucy=True, split=split)
+    value = n * steps / r["dt"]
+    prof = plain_phases(box, x, v, profile_steps, 4242, lucy=True, split=split)
+    f_ms, f_n = prof["force"]
+    t_force = f_ms / max(f_n, 1) * 1e-3
+    fk = B_FORCE * n / t_force / 1e9
+    return {
+        "workload": f"plain mDPD liquid slab: box {box[0]:g}x{box[1]:g}x{box[2]:g}, slab z in [5,15) at rho 6.05, "
+                    f"N={n}, A=-40 B=25 rc=1 rd=0.75 gamma=4.5 kT=1 dt=0.01, Lucy weight, skin 0.3, {steps} steps",
+        "particles": n, "steps": steps, "value": value, "split": split, "builds": r["nbuild"],
+        "step_algorithmic_GBps": value * B_STEP / 1e9, "step_hbm_frac": value * B_STEP / 1e9 / HBM_PEAK_GBS,
+        "force_kernel_GBps": fk, "force_kernel_frac": fk / HBM_PEAK_GBS, "force_kernel_us": t_force * 1e6,
+        "phase_ms_per_step": {k: ms / max(profile_steps, 1) for k, (ms, _) in prof.items()},
+        "final_temp": r["temp"],
+    }
 
 
 def base_line(a, value, ms_per_step, config, roof, cpu):
@@ -363,49 +568,74 @@ def base_line(a, value, ms_per_step, config, roof, cpu):
     }
 
 
-def run_batched(a):
+def run_headline(a, legs):
     sizes = batch_sizes(a.jobs, a.seed)
     specs = [(*make_droplet(int(n), a.seed + 1 + k), 5000 + k) for k, n in enumerate(sizes)]
     total = int(sizes.sum())
-    dt, st, temps = batched(specs, a.steps, a.warmup, a.split, a.budget_gb * GB, graph=not a.no_graph)
-    value = total * a.steps / dt
-    plain_value, plain_ns, plain_s = (plain_sample(sizes, lambda i: specs[i], a.steps, a.plain_sample)
-                                      if a.plain_sample > 0 else (None, [], 0.0))
-    srv = None
-    if a.served_seconds > 0:
-        srv, ssizes, sspec = served(a.served_seconds, a.served_rate, a.steps, a.served_budget_gb * GB, a.seed + 99,
-                                    split=a.served_split, graph=not a.no_graph)
-        if a.plain_sample > 0:
-            sv, sns, ss = plain_sample(ssizes, sspec, a.steps, max(4, a.plain_sample // 2))
-            srv.update({"plain_value": sv, "plain_sample_sizes": sns, "served_vs_plain": srv["value"] / sv})
-    t_force = st["prof_force_ms"] / max(st["prof_passes"], 1) * 1e-3
-    config = {
-        "workload": f"batched mDPD droplets: {a.jobs} periodic boxes, N ~ LogNormal(ln 8000, 0.5) clipped "
-                    f"[2000, 40000] (total {total}), droplet at rho 6.05 in a cube of side 2.5 R, A=-40 B=25 "
-                    f"rc=1 rd=0.75 gamma=4.5 kT=1 dt=0.01, quadratic weight, skin 0.3, {a.steps} steps each, "
-                    f"one pool (hipGraph-replayed passes, autotuned split)",
-        "mode": "batched", "jobs": a.jobs, "particles": total,
-        "plain_value": plain_value, "batched_vs_plain": value / plain_value if plain_value else None,
-        "plain_sample": f"{len(plain_ns)} of the same jobs (sizes {plain_ns}) on one handle each, "
-                        f"{a.steps} steps, {plain_s:.1f} s",
-        "passes": st["passes"], "split": st["split"], "pool_capacity": st["capacity"], "maxnbr": st["maxnbr"],
-        "pool_GB": st["bytes"] / GB, "regrows": st["regrows"],
-        "tune_first_force_ms_per_pass": {"1": st["tune_ms_1"], "2": st["tune_ms_2"], "4": st["tune_ms_4"]},
-        "sampled_ms_per_pass": {k: st[f"prof_{k}_ms"] / max(st["prof_passes"], 1)
-                                for k in ("rebuild", "first", "force", "commit")},
-        "sampled_passes": st["prof_passes"],
-        "step_algorithmic_GBps": value * B_STEP / 1e9,
-        "step_hbm_frac": value * B_STEP / 1e9 / HBM_PEAK_GBS,
-        "final_temp_sample": temps,
-        "served": srv,
-    }
-    roof = roofline(t_force, total, "k_force<mDPD, MODE_PHASED> (pool pass, all systems)", T_FORCE_POOL)
-    cpu = None
-    if not a.no_cpu:
+    config = {"workload": f"batched mDPD droplets: {a.jobs} periodic boxes, N ~ LogNormal(ln 8000, 0.5) clipped "
+                          f"[2000, 40000] (total {total}), droplet at rho 6.05 in a cube of side 2.5 R, A=-40 "
+                          f"B=25 rc=1 rd=0.75 gamma=4.5 kT=1 dt=0.01, quadratic weight, skin 0.3, "
+                          f"{a.job_steps} steps each, one pool (hipGraph-replayed passes, autotuned split)",
+              "mode": "batched", "jobs": a.jobs, "particles": total, "steps_per_job": a.job_steps}
+    value, roof, cpu, ms_per_step = None, None, None, None
+    if "batched" in legs:
+        dt, st, temps = batched(specs, a.job_steps, a.warmup, a.split, a.budget_gb * GB, graph=not a.no_graph)
+        value = total * a.job_steps / dt
+        ms_per_step = dt * 1e3 / a.steps
+        log(f"batched: {value:.3e} particle-steps/s over {dt:.2f} s")
+        med = int(np.argsort(sizes)[len(sizes) // 2])
+        box, x, v, seed = specs[med]
+        psplit, ptimes = plain_best_split(box, x, v, seed)
+        pv, pns, ps = (plain_sample(sizes, lambda i: specs[i], a.job_steps, a.plain_sample, psplit)
+                       if a.plain_sample > 0 else (None, [], 0.0))
+        log(f"plain: {pv:.3e} particle-steps/s (split {psplit})" if pv else "plain: skipped")
+        config.update({
+            "timed_region_s": dt, "ms_per_step_definition": "timed region / --steps (the region is the whole "
+                                                            "batch of 2000-step jobs)",
+            "passes": st["passes"], "ms_per_pass": dt * 1e3 / max(st["passes"], 1.0),
+            "plain_value": pv, "batched_vs_plain": value / pv if pv else None,
+            "plain_sample": f"{len(pns)} of the same jobs (sizes {pns}) on one handle each, {a.job_steps} steps, "
+                            f"fused steps replayed from a hipGraph with the reneighbor decision on the device, "
+                            f"split {psplit} (fastest over 200 steps of the median job: "
+                            f"{ {k: round(t, 4) for k, t in ptimes.items()} } s), {ps:.1f} s",
+            "split": st["split"], "pool_capacity": st["capacity"], "maxnbr": st["maxnbr"],
+            "pool_GB": st["bytes"] / GB, "regrows": st["regrows"],
+            "tune_first_force_ms_per_pass": {"1": st["tune_ms_1"], "2": st["tune_ms_2"], "4": st["tune_ms_4"]},
+            "sampled_ms_per_pass": sampled(st), "sampled_passes": st["prof_passes"],
+            "rebuild": rebuild_record(st, dt),
+            "step_algorithmic_GBps": value * B_STEP / 1e9,
+            "step_hbm_frac": value * B_STEP / 1e9 / HBM_PEAK_GBS,
+            "final_temp_sample": temps,
+        })
+        roof = force_roofline(st, "k_force<mDPD, MODE_PHASED> (batched pool pass)", T_FORCE_POOL)
+    if "slab" in legs:
+        config["slab"] = []
+        for name in ("slab-literal", "slab"):
+            log(f"slab: {name}")
+            config["slab"].append(slab_record(name, a.slab_steps))
+    if "cpu" in legs:
         med = int(np.argsort(sizes)[len(sizes) // 2])
         box, x, v, _ = specs[med]
-        cpu = cpu_baseline(box, x, v, False, "the median droplet of the batch")
-    print(json.dumps(base_line(a, value, dt / max(st["passes"], 1) * 1e3, config, roof, cpu)))
+        cpu = cpu_baselines(a, (box, x, v, "the median droplet of the batch"))
+    if "served" in legs:
+        srv, ssizes, sspec = served(a.served_seconds, a.served_rate, a.job_steps, a.served_budget_gb * GB,
+                                    a.seed + 99, a.served_max_wall, split=a.served_split, graph=not a.no_graph)
+        log(f"served: {srv['value']:.3e} particle-steps/s, p50 {srv['p50_latency_s']} s, p99 {srv['p99_latency_s']} s")
+        if a.plain_sample > 0:
+            k = max(4, a.plain_sample // 2)
+            order = np.argsort(ssizes, kind="stable")
+            mid = int(order[len(order) // 2])
+            box, x, v, seed = sspec(mid)
+            ssplit, _ = plain_best_split(box, x, v, seed)
+            sv, sns, ss = plain_sample(ssizes, sspec, a.job_steps, k, ssplit)
+            srv.update({"plain_value": sv, "plain_sample_sizes": sns, "plain_split": ssplit,
+                        "served_vs_plain": srv["value"] / sv})
+        config["served"] = srv
+        if value is None:
+            value = srv["value"]
+            ms_per_step = srv["wall_s"] * 1e3 / a.steps
+            config["mode"] = "served"
+    print(json.dumps(base_line(a, value, ms_per_step, config, roof, cpu)), flush=True)
 
 
 def run_pore(a):
@@ -444,85 +674,88 @@ def run_pore(a):
         s.close()
     s = handle()
     s.setup()
-    s.profile(True)
     t0 = time.perf_counter()
-    s.run(a.steps)
+    s.run(a.job_steps)
     dt = time.perf_counter() - t0
-    prof = {ph: s.profile_get(ph) for ph in ("force", "density", "rebuild", "initial")}
     vx = s.gather("v")[types == 2, 0].mean()
     nb = s.nbuild
     s.close()
-    value = n * a.steps / dt
+    s = handle()
+    s.setup()
+    s.profile(True)
+    s.run(200)
+    prof = {ph: s.profile_get(ph) for ph in ("force", "density", "rebuild", "initial")}
+    s.close()
+    value = n * a.job_steps / dt
     f_ms, f_n = prof["force"]
+    t_force = f_ms / max(f_n, 1) * 1e-3
     config = {
         "workload": f"plain mDPD quasi-2D pore flow: box 200x100x4, {ndisc} seeded non-overlapping discs "
                     f"(radius U(4,10)) covering 40% of the area, frozen wall particles (type 1, outside the NVE "
                     f"group, in the density) and fluid (type 2) at density 6, N={n} ({nwall} wall), A=-40, "
                     f"B=25 (solid-fluid B=0), gamma=4.5, rc=1, rd=0.75, g_x=0.02 on the fluid, dt=0.01, "
-                    f"skin 0.3, one system per launch",
-        "mode": "plain", "particles": n, "wall_particles": nwall, "builds": nb,
-        "phase_ms_per_step": {k: ms / max(a.steps, 1) for k, (ms, _) in prof.items()},
+                    f"skin 0.3, one system per launch, {a.job_steps} steps",
+        "mode": "plain", "particles": n, "wall_particles": nwall, "builds": nb, "timed_region_s": dt,
+        "phase_ms_per_step": {k: ms / 200 for k, (ms, _) in prof.items()},
         "fluid_mean_vx_end": float(vx),
         "step_algorithmic_GBps": value * B_STEP / 1e9,
     }
-    roof = roofline(f_ms / max(f_n, 1) * 1e-3, n, "k_force<mDPD, fused velocity-Verlet, 2 types>")
+    achieved = B_FORCE * n / t_force / 1e9
+    roof = {"bound": "hbm", "kernel": "k_force<mDPD, fused velocity-Verlet, 2 types>", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_particle": B_FORCE, "particles_per_launch": n, "avg_launch_us": t_force * 1e6}
     print(json.dumps(base_line(a, value, dt / a.steps * 1e3, config, roof, None)))
 
 
 def run_dpd(a):
     """Config 1: the classic DPD fluid (L = 22, N = 32,000, a = 25, gamma = 4.5,
     dt = 0.04) on the GPU (dpd/meso, sigma = 3) beside the CPU LAMMPS
-    pair_style dpd + fix nve transcription (serial: one Marsaglia stream)."""
+    pair_style dpd + fix nve transcription."""
     import mdpd355
-    sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import orc
     L, n = 22.0, 32000
-    rng = np.random.default_rng(a.seed)
+    rng = np.random.default_rng(20241203)
     x = _f32(rng.uniform(0, L, (n, 3)))
     v = _f32(_mb(rng, n))
-    s = mdpd355.System(0)
-    s.command("atom_style dpd/atomic/meso")
-    s.create_box(1, (0, 0, 0), (L, L, L))
-    s.create_atoms(np.ones(n, np.int32), x, v)
-    s.commands("""
-        neighbor 0.3 bin
-        neigh_modify every 1 delay 0 check yes
-        mass 1 1.0
-        pair_style dpd/meso 1.0 4242
-        pair_coeff * * 25.0 4.5 3.0 1.0 1.0
-        fix int all nve/meso
-        timestep 0.04
-        run_style mvv/meso
-    """)
+
+    def handle():
+        s = mdpd355.System(0)
+        s.command("atom_style dpd/atomic/meso")
+        s.create_box(1, (0, 0, 0), (L, L, L))
+        s.create_atoms(np.ones(n, np.int32), x, v)
+        s.commands("""
+            neighbor 0.3 bin
+            neigh_modify every 1 delay 0 check yes
+            mass 1 1.0
+            pair_style dpd/meso 1.0 4242
+            pair_coeff * * 25.0 4.5 3.0 1.0 1.0
+            fix int all nve/meso
+            timestep 0.04
+            run_style mvv/meso
+        """)
+        return s
+
+    s = handle()
     s.run(a.warmup)
-    s.profile(True)
     t0 = time.perf_counter()
-    s.run(a.steps)
+    s.run(1000)
     dt = time.perf_counter() - t0
-    prof = {ph: s.profile_get(ph) for ph in ("force", "density", "rebuild", "initial")}
     th = s.thermo()
     s.close()
-    value = n * a.steps / dt
-    cpu = None
-    if not a.no_cpu:
-        steps = 2
-        t0 = time.perf_counter()
-        orc.lammps_dpd_run(x, v, (0, 0, 0), (L, L, L), nsteps=steps, dt=0.04, seed=4321)
-        per = (time.perf_counter() - t0) / steps
-        steps = int(max(2, min(1000, 12.0 / per)))
-        t0 = time.perf_counter()
-        out = orc.lammps_dpd_run(x, v, (0, 0, 0), (L, L, L), nsteps=steps, dt=0.04, seed=4321)
-        t = time.perf_counter() - t0
-        cpu = {"value": n * steps / t, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"LAMMPS pair_style dpd + fix nve transcription (FP64, half list, serial RanMars) on "
-                         f"the same N={n} fluid, {steps} steps incl. the initial build, {t:.1f} s wall"}
+    s = handle()
+    s.setup()
+    s.profile(True)
+    s.run(200)
+    prof = {ph: s.profile_get(ph) for ph in ("force", "density", "rebuild", "initial")}
+    s.close()
+    value = n * 1000 / dt
+    cpu = None if a.no_cpu else cpu_baselines(a)
     b_dpd = 56 + 4 * 3.0 * 4.0 / 3.0 * np.pi * 1.3**3
     f_ms, f_n = prof["force"]
     config = {
         "workload": f"plain DPD fluid (config 1): periodic cube L=22, N={n} uniform, a=25 gamma=4.5 sigma=3 "
-                    f"rc=1 dt=0.04, skin 0.3, dpd/meso on one handle",
+                    f"rc=1 dt=0.04, skin 0.3, dpd/meso on one handle, 1000 steps",
         "mode": "plain", "particles": n, "final_temp": th["temp"], "final_press": th["press"],
-        "phase_ms_per_step": {k: ms / max(a.steps, 1) for k, (ms, _) in prof.items()},
+        "timed_region_s": dt, "phase_ms_per_step": {k: ms / 200 for k, (ms, _) in prof.items()},
     }
     roof = {"bound": "hbm", "kernel": "k_force<DPD, fused velocity-Verlet>",
             "achieved": b_dpd * n / (f_ms / max(f_n, 1) * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -532,50 +765,49 @@ def run_dpd(a):
 
 
 def run_slab(a):
+    rec = slab_record(a.workload, a.slab_steps, split=a.split)
     w = SLABS[a.workload]
-    box = w["box"]
-    x, v = make_slab(box, w["slab"])
-    n = len(x)
-    if a.warmup > 0:
-        plain_job(box, x, v, a.warmup, 4242, lucy=True, split=a.split)
-    r = plain_job(box, x, v, a.steps, 4242, lucy=True, split=a.split, profile=True)
-    dt, prof = r["dt"], r["prof"]
-    value = n * a.steps / dt
-    f_ms, f_n = prof["force"]
-    config = {
-        "workload": f"plain mDPD liquid slab ({a.workload}): box {box[0]:g}x{box[1]:g}x{box[2]:g}, slab z in "
-                    f"[5,15) at rho 6.05, N={n}, A=-40 B=25 rc=1 rd=0.75 gamma=4.5 kT=1 dt=0.01, Lucy weight, "
-                    f"skin 0.3, one system per launch",
-        "mode": "plain", "particles": n, "rebuilds_in_timed_region": r["nbuild"],
-        "phase_ms_per_step": {k: ms / max(a.steps, 1) for k, (ms, _) in prof.items()},
-        "step_algorithmic_GBps": value * B_STEP / 1e9,
-        "step_hbm_frac": value * B_STEP / 1e9 / HBM_PEAK_GBS,
-        "final_temp": r["thermo"]["temp"],
-    }
-    roof = roofline(f_ms / max(f_n, 1) * 1e-3, n, "k_force<mDPD, fused velocity-Verlet>")
-    cpu = None if a.no_cpu else cpu_baseline(box, x, v, True, "the same slab")
-    print(json.dumps(base_line(a, value, dt / a.steps * 1e3, config, roof, cpu)))
+    cpu = None
+    if not a.no_cpu:
+        x, v = make_slab(w["box"], w["slab"])
+        cpu = cpu_mdpd(w["box"], x, v, True, "the same slab", a.cpu_seconds, max(1, host_cpus()["physical_cores"]))
+    roof = {"bound": "hbm", "kernel": "k_force<mDPD, fused velocity-Verlet>", "achieved": rec["force_kernel_GBps"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rec["force_kernel_frac"], "traffic": None,
+            "bytes_per_particle": B_FORCE, "particles_per_launch": rec["particles"],
+            "avg_launch_us": rec["force_kernel_us"]}
+    config = dict(rec)
+    config["mode"] = "plain"
+    print(json.dumps(base_line(a, rec["value"], rec["particles"] * rec["steps"] / rec["value"] * 1e3 / a.steps,
+                               config, roof, cpu)))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000, help="steps per job")
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="batched", choices=["batched", "pore", "dpd"] + sorted(SLABS))
+    ap.add_argument("--steps", type=int, default=20, help="K of the contract: ms_per_step = timed region / K")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed passes of a warm-up pool")
+    ap.add_argument("--job-steps", type=int, default=2000, help="steps of every batched / served job")
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "batched", "served", "pore", "dpd"] + sorted(SLABS))
+    ap.add_argument("--legs", default="batched,slab,cpu,served",
+                    help="legs of the headline workload (batched, slab, cpu, served)")
     ap.add_argument("--jobs", type=int, default=256)
     ap.add_argument("--seed", type=int, default=20241202)
     ap.add_argument("--budget-gb", type=float, default=16.0, help="HBM budget of the batched pool")
     ap.add_argument("--plain-sample", type=int, default=16, help="jobs of the batch timed on the plain path")
-    ap.add_argument("--served-seconds", type=float, default=10.0, help="arrival window of the served leg (0: skip)")
+    ap.add_argument("--served-seconds", type=float, default=120.0, help="arrival window of the served leg")
     ap.add_argument("--served-rate", type=float, default=50.0)
     ap.add_argument("--served-budget-gb", type=float, default=4.0)
+    ap.add_argument("--served-max-wall", type=float, default=300.0,
+                    help="the served leg stops here (reported as truncated) so that the run keeps its time limit")
     ap.add_argument("--served-split", type=int, default=0, choices=[0, 1, 2, 4],
                     help="waves per tile of the served pool (0: autotuned)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--slab-steps", type=int, default=5000)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work per baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched and served pools: launch each pass's kernels directly instead of replaying a hipGraph")
-    ap.add_argument("--split", type=int, default=None, help="waves per tile (default: autotuned / 2)")
+    ap.add_argument("--split", type=int, default=None, help="waves per tile (default: autotuned)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device with a plain copy kernel (a known-size "
                          "reference for HBM byte counters under rocprofv3 --pmc)")
@@ -592,8 +824,15 @@ def main():
         torch.cuda.synchronize()
         print(f"copied {a.calibrate_copy_mib} MiB", file=sys.stderr)
         del src, dst
-    if a.workload == "batched":
-        run_batched(a)
+    if a.workload == "headline":
+        legs = set(a.legs.split(","))
+        if a.no_cpu:
+            legs.discard("cpu")
+        run_headline(a, legs)
+    elif a.workload == "batched":
+        run_headline(a, {"batched"} | (set() if a.no_cpu else {"cpu"}))
+    elif a.workload == "served":
+        run_headline(a, {"served"})
     elif a.workload == "pore":
         run_pore(a)
     elif a.workload == "dpd":

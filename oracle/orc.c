@@ -871,31 +871,114 @@ double orc_ranmars_gaussian(orc_ranmars *r)
     return first;
 }
 
+/* Pair forces of the half list (newton on: f_j -= F), pair_dpd.cpp:63-160.
+ * Thread t of nt accumulates into its own force array with its own Marsaglia
+ * stream; the arrays are summed afterwards.  With nt = 1 this is the serial
+ * loop with the one stream of PairDPD::settings. */
+typedef struct {
+    int n, nt, nrdf;
+    const double *L;
+    double a0, gamma, sigma, rc, rc2, dtinvsqrt;
+    orc_ranmars *rng;     /* nt streams */
+    double *fthr;         /* nt * 3n (thread 0 is the output f) */
+    double *virthr;       /* nt */
+    double *rdfthr;       /* nt * nrdf, or NULL */
+} dpd_half_ctx;
+
+static void dpd_half_forces(dpd_half_ctx *c, const double *x, const double *v, const int *wk, const nlist_t *nl,
+                            int sample)
+{
+    const size_t n3 = 3 * (size_t)c->n;
+#pragma omp parallel num_threads(c->nt)
+    {
+        const int t = c->nt > 1 ? omp_get_thread_num() : 0;
+        double *f = c->fthr + (size_t)t * n3;
+        orc_ranmars *rng = &c->rng[t];
+        double *rdf = c->rdfthr ? c->rdfthr + (size_t)t * c->nrdf : NULL;
+        double virial = 0;
+        memset(f, 0, sizeof(double) * n3);
+#pragma omp for schedule(static)
+        for (int i = 0; i < c->n; ++i) {
+            for (int k = nl->start[i]; k < nl->start[i + 1]; ++k) {
+                int j = nl->j[k];
+                if (j <= i) continue;
+                double d[3];
+                pair_dx(x, wk, c->L, i, j, nl->code[k], d);
+                double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (rsq >= c->rc2) continue;
+                double r = sqrt(rsq);
+                if (r < 1e-10) continue;
+                double rinv = 1.0 / r;
+                double dvx = v[3 * i] - v[3 * j];
+                double dvy = v[3 * i + 1] - v[3 * j + 1];
+                double dvz = v[3 * i + 2] - v[3 * j + 2];
+                double dot = d[0] * dvx + d[1] * dvy + d[2] * dvz;
+                double wd = 1.0 - r / c->rc;
+                double rn = orc_ranmars_gaussian(rng);
+                double fp = c->a0 * wd;
+                fp -= c->gamma * wd * wd * dot * rinv;
+                fp += c->sigma * wd * rn * c->dtinvsqrt;
+                fp *= rinv;
+                for (int a = 0; a < 3; ++a) {
+                    f[3 * i + a] += d[a] * fp;
+                    f[3 * j + a] -= d[a] * fp;
+                }
+                if (sample) virial += rsq * fp;
+                if (sample && rdf) {
+                    int b = (int)(r / c->rc * c->nrdf);
+                    if (b < c->nrdf) rdf[b] += 1.0;
+                }
+            }
+        }
+        c->virthr[t] = virial;
+        if (c->nt > 1) {
+            /* sum the thread arrays into thread 0's (the implicit barrier of
+             * the loop above has passed) */
+#pragma omp for schedule(static)
+            for (size_t a = 0; a < n3; ++a) {
+                double s = c->fthr[a];
+                for (int q = 1; q < c->nt; ++q) s += c->fthr[(size_t)q * n3 + a];
+                c->fthr[a] = s;
+            }
+        }
+    }
+}
+
 int orc_lammps_dpd_run(int n, double *x, double *v, const double lo[3], const double hi[3],
                        double a0, double gamma, double temperature, double rc,
                        double skin, double dt, int seed, int nsteps, int equil,
                        int sample_every, double *t_sum, double *p_sum, int *nsamples,
-                       int nrdf, double *rdf_hist)
+                       int nrdf, double *rdf_hist, int nthreads)
 {
     /* PairDPD::settings: RanMars(seed + me) with me = 0 (pair_dpd.cpp:198);
+     * with nthreads > 1 thread t draws from RanMars(seed + t), as LAMMPS's
+     * OpenMP variant of the pair style seeds its per-thread streams;
      * init_one: sigma = sqrt(2 kT gamma) (pair_dpd.cpp:262) */
-    orc_ranmars rng;
-    orc_ranmars_init(&rng, seed);
-    const double sigma = sqrt(2.0 * temperature * gamma);
-    const double dtinvsqrt = 1.0 / sqrt(dt), dtf = 0.5 * dt;
-    const double rc2 = rc * rc, trig = 0.25 * skin * skin;
+    const int nt = nthreads > 1 ? nthreads : 1;
+    const int prev_threads = omp_get_max_threads();
+    if (nthreads > 0) omp_set_num_threads(nthreads);   /* the neighbor build too */
+    dpd_half_ctx c;
+    c.n = n; c.nt = nt; c.nrdf = nrdf;
+    c.a0 = a0; c.gamma = gamma; c.sigma = sqrt(2.0 * temperature * gamma); c.rc = rc; c.rc2 = rc * rc;
+    c.dtinvsqrt = 1.0 / sqrt(dt);
     double L[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
-    double V = L[0] * L[1] * L[2];
+    c.L = L;
+    c.rng = (orc_ranmars *)malloc(sizeof(orc_ranmars) * (size_t)nt);
+    for (int t = 0; t < nt; ++t) orc_ranmars_init(&c.rng[t], seed + t);
     size_t n3 = 3 * (size_t)n;
-    double *f = (double *)calloc(n3, sizeof(double));
-    double *xhold = (double *)malloc(sizeof(double) * n3);
-    int *wk = (int *)malloc(sizeof(int) * n3);
+    c.fthr = (double *)calloc(n3 * (size_t)nt + 1, sizeof(double));
+    c.virthr = (double *)calloc((size_t)nt, sizeof(double));
+    c.rdfthr = (rdf_hist && nrdf > 0) ? (double *)calloc((size_t)nt * (size_t)nrdf, sizeof(double)) : NULL;
+    double *f = c.fthr;
+    const double dtf = 0.5 * dt, trig = 0.25 * skin * skin;
+    double V = L[0] * L[1] * L[2];
+    double *xhold = (double *)malloc(sizeof(double) * (n3 + 1));
+    int *wk = (int *)malloc(sizeof(int) * (n3 + 1));
     nlist_t nl = {0};
     int nbuild = 0;
-    double virial = 0;
 
-    /* half list: keep (i, j, code) with j > i, or j == i never; the partner
-     * entry (j, i, 26-code) is dropped.  Newton on: f_j -= F. */
+    /* the full list of nlist_build, used as a half list: entry (i, j) with
+     * j > i; its partner (j, i) is skipped */
 #define BUILD_HALF()                                                           \
     do {                                                                       \
         nlist_free(&nl);                                                       \
@@ -905,65 +988,30 @@ int orc_lammps_dpd_run(int n, double *x, double *v, const double lo[3], const do
         ++nbuild;                                                              \
     } while (0)
 
-#define COMPUTE_FORCES(do_sample)                                              \
-    do {                                                                       \
-        memset(f, 0, sizeof(double) * n3);                                     \
-        virial = 0;                                                            \
-        for (int i = 0; i < n; ++i) {                                          \
-            for (int k = nl.start[i]; k < nl.start[i + 1]; ++k) {              \
-                int j = nl.j[k];                                               \
-                if (j <= i) continue;                                          \
-                double d[3];                                                   \
-                pair_dx(x, wk, L, i, j, nl.code[k], d);                        \
-                double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];          \
-                if (rsq < rc2) {                                               \
-                    double r = sqrt(rsq);                                      \
-                    if (r < 1e-10) continue;                                   \
-                    double rinv = 1.0 / r;                                     \
-                    double dvx = v[3 * i] - v[3 * j];                          \
-                    double dvy = v[3 * i + 1] - v[3 * j + 1];                  \
-                    double dvz = v[3 * i + 2] - v[3 * j + 2];                  \
-                    double dot = d[0] * dvx + d[1] * dvy + d[2] * dvz;         \
-                    double wd = 1.0 - r / rc;                                  \
-                    double rn = orc_ranmars_gaussian(&rng);                    \
-                    double fp = a0 * wd;                                       \
-                    fp -= gamma * wd * wd * dot * rinv;                        \
-                    fp += sigma * wd * rn * dtinvsqrt;                         \
-                    fp *= rinv;                                                \
-                    for (int a = 0; a < 3; ++a) {                              \
-                        f[3 * i + a] += d[a] * fp;                             \
-                        f[3 * j + a] -= d[a] * fp;                             \
-                    }                                                          \
-                    if (do_sample) virial += rsq * fp;                         \
-                    if (do_sample && rdf_hist && nrdf > 0) {                   \
-                        int b = (int)(r / rc * nrdf);                          \
-                        if (b < nrdf) rdf_hist[b] += 1.0;                      \
-                    }                                                          \
-                }                                                              \
-            }                                                                  \
-        }                                                                      \
-    } while (0)
-
     BUILD_HALF();
-    COMPUTE_FORCES(0);
+    dpd_half_forces(&c, x, v, wk, &nl, 0);
     *t_sum = 0; *p_sum = 0; *nsamples = 0;
     for (int step = 1; step <= nsteps; ++step) {
+#pragma omp parallel for schedule(static) num_threads(nt)
         for (size_t a = 0; a < n3; ++a) {              /* fix_nve.cpp:64-95 */
             v[a] += dtf * f[a];
             x[a] += dt * v[a];
         }
         int flag = 0;
-        for (int i = 0; i < n && !flag; ++i) {
+#pragma omp parallel for schedule(static) num_threads(nt) reduction(| : flag)
+        for (int i = 0; i < n; ++i) {
             double dx = x[3 * i] - xhold[3 * i], dy = x[3 * i + 1] - xhold[3 * i + 1], dz = x[3 * i + 2] - xhold[3 * i + 2];
-            if (dx * dx + dy * dy + dz * dz > trig) flag = 1;
+            if (dx * dx + dy * dy + dz * dz > trig) flag |= 1;
         }
         if (flag) BUILD_HALF();
         int sample = (step > equil) && sample_every > 0 && ((step - equil) % sample_every == 0);
-        COMPUTE_FORCES(sample);
+        dpd_half_forces(&c, x, v, wk, &nl, sample);
+#pragma omp parallel for schedule(static) num_threads(nt)
         for (size_t a = 0; a < n3; ++a) v[a] += dtf * f[a];   /* fix_nve.cpp:100-130 */
         if (sample) {
-            double ke = 0;
+            double ke = 0, virial = 0;
             for (size_t a = 0; a < n3; ++a) ke += v[a] * v[a];
+            for (int t = 0; t < nt; ++t) virial += c.virthr[t];
             double dof = 3.0 * n - 3.0;
             double T = ke / dof;
             *t_sum += T;
@@ -972,9 +1020,13 @@ int orc_lammps_dpd_run(int n, double *x, double *v, const double lo[3], const do
         }
     }
 #undef BUILD_HALF
-#undef COMPUTE_FORCES
+    if (c.rdfthr)
+        for (int t = 0; t < nt; ++t)
+            for (int b = 0; b < nrdf; ++b) rdf_hist[b] += c.rdfthr[(size_t)t * nrdf + b];
     nlist_free(&nl);
-    free(f); free(xhold); free(wk);
+    free(c.fthr); free(c.virthr); free(c.rdfthr); free(c.rng);
+    free(xhold); free(wk);
+    omp_set_num_threads(prev_threads);
     return nbuild;
 }
 

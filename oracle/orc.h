@@ -202,16 +202,20 @@ void   orc_ranmars_init(orc_ranmars *r, int seed);
 double orc_ranmars_uniform(orc_ranmars *r);
 double orc_ranmars_gaussian(orc_ranmars *r);
 
-/* LAMMPS pair_style dpd + fix nve + Verlet, half neighbor list, newton on,
- * serial Marsaglia stream; src/pair_dpd.cpp:63-160, src/fix_nve.cpp:64-130,
- * src/verlet.cpp:212+.  Thermo accumulators: sum of T and P over steps
- * sampled every `sample_every` steps after `equil` steps; g(r) histogram with
- * nrdf bins over [0, rc).  Returns number of rebuilds. */
+/* LAMMPS pair_style dpd + fix nve + Verlet, half neighbor list, newton on;
+ * src/pair_dpd.cpp:63-160, src/fix_nve.cpp:64-130, src/verlet.cpp:212+.
+ * nthreads <= 1: one serial Marsaglia stream (pair_dpd.cpp:198); nthreads > 1:
+ * OpenMP threads with a force array and a stream RanMars(seed + t) each,
+ * summed after the pair loop (the neighbor build and integration also use
+ * nthreads; 0 leaves the OpenMP default for the build).  Thermo
+ * accumulators: sum of T and P over steps sampled every `sample_every` steps
+ * after `equil` steps; g(r) histogram with nrdf bins over [0, rc).  Returns
+ * number of rebuilds. */
 int orc_lammps_dpd_run(int n, double *x, double *v, const double lo[3], const double hi[3],
                        double a0, double gamma, double temperature, double rc,
                        double skin, double dt, int seed, int nsteps, int equil,
                        int sample_every, double *t_sum, double *p_sum, int *nsamples,
-                       int nrdf, double *rdf_hist);
+                       int nrdf, double *rdf_hist, int nthreads);
 
 /* observables; compute_temp_meso.cu:49-101 (dof = 3N - 3),
  * compute_pressure_meso.cu:115-155 */

@@ -90,7 +90,7 @@ def _declare(L):
         "orc_ranmars_init": (None, [P, I]),
         "orc_ranmars_uniform": (D, [P]),
         "orc_ranmars_gaussian": (D, [P]),
-        "orc_lammps_dpd_run": (I, [I, P, P, P, P, D, D, D, D, D, D, I, I, I, I, P, P, P, I, P]),
+        "orc_lammps_dpd_run": (I, [I, P, P, P, P, D, D, D, D, D, D, I, I, I, I, P, P, P, I, P, I]),
         "orc_temperature": (D, [I, P, P, P]),
         "orc_pressure": (D, [I, P, P, P, P]),
         "orc_premix_n": (None, [I, P, P, I, P]),
@@ -244,6 +244,21 @@ def force_dpd(x, v, sig, types0, p):
     return f, e, vir
 
 
+def temperature(v, types0, p):
+    """compute temp/meso of FP64 velocities (dof = 3N - 3)."""
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(types0, dtype=np.int32)
+    return lib().orc_temperature(v.shape[0], _p(v), _p(t), ctypes.byref(p))
+
+
+def pressure(v, vir, types0, p):
+    """compute pressure/meso from velocities and per-atom virials (n x 6)."""
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(vir, dtype=np.float64).reshape(-1, 6)
+    t = np.ascontiguousarray(types0, dtype=np.int32)
+    return lib().orc_pressure(v.shape[0], _p(v), _p(w), _p(t), ctypes.byref(p))
+
+
 def arb(mobile_bit, wall_bit, cut_phi=1.0, phi_c=0.5, rho_wall=8.0, pair=True, fix=True):
     """Arbitrary-boundary settings: the same mobile / wall groups for the pair
     style's wall scaling and for the fix's wall fraction + bounce-back."""
@@ -331,7 +346,7 @@ def ranmars_gaussian(r, k=1):
 
 
 def lammps_dpd_run(x, v, lo, hi, a0=25.0, gamma=4.5, temperature=1.0, rc=1.0, skin=0.3, dt=0.04,
-                   seed=12345, nsteps=100, equil=0, sample_every=10, nrdf=50):
+                   seed=12345, nsteps=100, equil=0, sample_every=10, nrdf=50, nthreads=0):
     x = np.array(x, dtype=np.float64).reshape(-1, 3)
     v = np.array(v, dtype=np.float64).reshape(-1, 3)
     n = x.shape[0]
@@ -340,7 +355,8 @@ def lammps_dpd_run(x, v, lo, hi, a0=25.0, gamma=4.5, temperature=1.0, rc=1.0, sk
     hist = np.zeros(nrdf)
     nb = lib().orc_lammps_dpd_run(n, _p(x), _p(v), _p(_d3(lo)), _p(_d3(hi)), a0, gamma, temperature, rc,
                                   skin, dt, int(seed), int(nsteps), int(equil), int(sample_every),
-                                  ctypes.byref(ts), ctypes.byref(ps), ctypes.byref(ns), nrdf, _p(hist))
+                                  ctypes.byref(ts), ctypes.byref(ps), ctypes.byref(ns), nrdf, _p(hist),
+                                  int(nthreads))
     k = max(ns.value, 1)
     return {"x": x, "v": v, "T": ts.value / k, "P": ps.value / k, "nsamples": ns.value,
             "rdf_hist": hist, "nbuild": nb}

@@ -7,7 +7,7 @@ wall fraction and bounce-back), read through read_data.
 
 The deck's fluid density profile along y (10 bins, counts per bin, time
 averaged) is compared with the reference's own outputs for two wall
-wettabilities A_12 = -20 and -35 (GPU run out.profile.a_*, CPU run
+wettabilities A_12 = -20, -30 and -35 (GPU run out.profile.a_*, CPU run
 out.cpu.a*; the two references agree to about 1%).  The engine runs the
 5,000-step pre-run, then averages the profile over 30,000 steps (the
 reference: 100,000); bins inside the channel must agree with the reference
@@ -60,7 +60,7 @@ def deck(a12):
     """
 
 
-@pytest.mark.parametrize("a12", [20, 35])
+@pytest.mark.parametrize("a12", [20, 30, 35])
 def test_contact_angle_profile_matches_reference(engine, tmp_path, a12):
     g = np.load(FIX)
     data = tmp_path / "init.data"

@@ -357,29 +357,6 @@ def test_short_trajectory_tracks_oracle(engine, orc):
     assert np.abs(dx).max() < 1e-3
 
 
-def test_frozen_walls_and_body_force(engine, orc):
-    L = 8.0
-    x, v = uniform_box(2000, L, 43)
-    types = np.where(x[:, 2] < 2.0, 2, 1).astype(np.int32)
-    s = mdpd_system(engine, x, v, L, types=types, ntypes=2, extra="")
-    s.commands("""
-        group fluid type 1
-        group wall type 2
-        unfix int
-        fix int fluid nve/meso
-        fix g fluid addforce/meso 0.02 0.0 0.0
-    """)
-    s.setup()
-    x0 = s.gather("x")
-    s.run(20)
-    x1 = s.gather("x")
-    wall = types == 2
-    assert np.array_equal(x1[wall], x0[wall])
-    assert not np.allclose(x1[~wall], x0[~wall])
-    # body force shows up in f of fluid particles only (compare with setup)
-    p = oracle_params(orc, L)
-
-
 def test_momentum_conserved_over_run(engine):
     L = 8.0
     x, v = uniform_box(2400, L, 47)

@@ -302,6 +302,68 @@ def test_dpd_energy_is_potential_of_force(orc):
         assert f[1, 0] == pytest.approx(-dE, rel=2e-3)
 
 
+def test_dpd_weight_exponent_two_particles(orc):
+    # dpd/meso with w_r = w_c^expw (pair_dpd_meso.cu:153): for two particles
+    # on the x axis with a relative velocity, F_i = [a0 w_c - gamma w_r^2
+    # (dx.dv)/r + sigma w_r xi / sqrt(dt)] dx / r, written out by hand here
+    # with xi from the (separately pinned) pair Gaussian of the signatures
+    L, dt, a0, g, sg = 10.0, 0.04, 25.0, 4.5, 3.0
+    sig = [0x12345678, 0x9ABCDEF0]
+    xi = float(orc.gaussian([sig[0]], [sig[1]])[0])
+    for expw in (1.0, 0.5, 0.25):
+        p = orc.params((0, 0, 0), (L, L, L), dt, cut=1.0, A=a0, gamma=g, sigma=sg, expw=expw)
+        for r, dv in ((0.3, 0.7), (0.85, -1.2)):
+            x = np.array([[5.0, 5.0, 5.0], [5.0 + r, 5.0, 5.0]], dtype=np.float32)
+            v = np.array([[dv, 0.0, 0.0], [0.0, 0.0, 0.0]], dtype=np.float32)
+            rr = float(np.float32(5.0 + r) - np.float32(5.0))
+            dvx = float(np.float32(dv))
+            wc = 1.0 - rr
+            wr = wc ** expw
+            dx = -rr                                   # x_i - x_j
+            fpair = (a0 * wc - g * wr * wr * (dx * dvx) / rr + sg * wr * xi / math.sqrt(dt)) / rr
+            f, e, _ = orc.force_dpd(x, v, sig, [0, 0], p)
+            assert f[0, 0] == pytest.approx(fpair * dx, rel=1e-12), (expw, r)
+            assert f[1, 0] == pytest.approx(-fpair * dx, rel=1e-12), (expw, r)
+            assert np.allclose(f[:, 1:], 0)
+            assert e.sum() == pytest.approx(0.5 * a0 * wc * wc, rel=1e-12)   # two halves of 0.5 a0 rc wc^2
+
+
+# ---------------------------------------------------------------- observables
+
+def test_temperature_hand_values(orc):
+    # compute temp/meso (compute_temp_meso.cu:49-101): sum m v^2 / dof with
+    # dof = 3N - 3, here for four particles of two masses
+    L = 10.0
+    p = orc.params((0, 0, 0), (L, L, L), 0.01, mass=(1.0, 2.0), ntypes=2, cut=1.0)
+    v = [[1.0, 0.0, 0.0], [0.0, 2.0, 0.0], [0.0, 0.0, -1.0], [1.0, 1.0, 1.0]]
+    t = [0, 1, 0, 1]
+    # 1*1 + 2*4 + 1*1 + 2*3 = 16 over dof 9
+    assert orc.temperature(v, t, p) == pytest.approx(16.0 / 9.0, rel=1e-15)
+    assert orc.temperature([[3.0, 0.0, 0.0]], [0], p) == 0.0     # one particle: no dof
+
+
+def test_pressure_ideal_and_pair_virial(orc):
+    # compute pressure/meso (compute_pressure_meso.cu:115-155):
+    # P = (dof T + sum of virial traces) / (3 V)
+    L = 10.0
+    V = L**3
+    p = orc.params((0, 0, 0), (L, L, L), 0.01, mass=(1.0, 2.0), ntypes=2, cut=1.0)
+    v = np.array([[1.0, 0.0, 0.0], [0.0, 2.0, 0.0], [0.0, 0.0, -1.0], [1.0, 1.0, 1.0]])
+    t = [0, 1, 0, 1]
+    # no virial: P V = sum m v^2 / 3 (the kinetic part, whatever dof is)
+    assert orc.pressure(v, np.zeros((4, 6)), t, p) == pytest.approx(16.0 / 3.0 / V, rel=1e-15)
+    # at rest, a conservative mDPD pair at distance r on the x axis: the
+    # virial is r F(r) with the hand-evaluated pair force
+    pm = _p(orc, L, gamma=0.0, sigma=0.0)
+    r = 0.5
+    x = np.array([[5.0, 5.0, 5.0], [5.0 + r, 5.0, 5.0]], dtype=np.float32)
+    rho = np.array([4.0, 6.0])
+    _, _, vir = orc.force_mdpd(x, np.zeros((2, 3)), rho, [1, 2], [0, 0], pm)
+    rr = float(np.float32(5.0 + r) - np.float32(5.0))
+    fmag = -40.0 * (1 - rr) + 25.0 * 10.0 * max(1 - rr / RD, 0.0)
+    assert orc.pressure(np.zeros((2, 3)), vir, [0, 0], pm) == pytest.approx(rr * fmag / 3.0 / V, rel=1e-12)
+
+
 # ---------------------------------------------------------------- integrator
 
 def test_free_flight_and_parabola(orc):
@@ -332,6 +394,22 @@ def test_lammps_dpd_statistics(orc):
     x, v = uniform_box(n, L, 51)
     out = orc.lammps_dpd_run(x, v, (0, 0, 0), (L, L, L), a0=25.0, gamma=4.5, dt=0.04, seed=4321,
                              nsteps=400, equil=150, sample_every=5)
+    assert out["T"] == pytest.approx(1.0, abs=0.05)
+    alpha = (out["P"] - 3.0 * out["T"]) / (25.0 * 9.0)
+    assert 0.085 < alpha < 0.1047
+
+
+def test_lammps_dpd_threads(orc):
+    # one thread is the serial run bit for bit (one stream, same order);
+    # four threads draw from four streams and give the same statistics
+    L = 9.0
+    n = int(3 * L**3)
+    x, v = uniform_box(n, L, 53)
+    kw = dict(a0=25.0, gamma=4.5, dt=0.04, seed=4321)
+    a = orc.lammps_dpd_run(x, v, (0, 0, 0), (L, L, L), nsteps=30, **kw)
+    b = orc.lammps_dpd_run(x, v, (0, 0, 0), (L, L, L), nsteps=30, nthreads=1, **kw)
+    assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["v"], b["v"])
+    out = orc.lammps_dpd_run(x, v, (0, 0, 0), (L, L, L), nsteps=400, equil=150, sample_every=5, nthreads=4, **kw)
     assert out["T"] == pytest.approx(1.0, abs=0.05)
     alpha = (out["P"] - 3.0 * out["T"]) / (25.0 * 9.0)
     assert 0.085 < alpha < 0.1047

@@ -238,3 +238,27 @@ def test_job_finished_before_a_regrow_keeps_its_last_pass(engine):
     got = ({k: p.gather(ja, k) for k in ("x", "v", "f", "image", "e_pair", "virial")}, p.thermo(ja),
            p.info(ja)["timestep"])
     assert_same(got, alone, "job finished before a regrow")
+
+
+def test_split_change_before_a_harvest_keeps_the_last_pass(engine):
+    # job a finishes at split 1 while job b runs on; the split then changes
+    # to 4 before a's status is handled.  b's passes at split 4 must not
+    # touch the compact-row counts a's harvest pass (energy, virial, f of its
+    # last pass) reads: a's results equal a alone at split 1, bit for bit
+    a_sys = lambda: droplet(engine, 2000, 62)
+    alone = pool_results(engine, [a_sys()], [3], split=1, poll=8)[0][0]
+    p = engine.Pool(0.5 * GB, 4)
+    p.tune("split", 1)
+    p.tune("poll", 8)
+    # b first: its tiles come first, so its split-4 count slots would cover
+    # a's tiles if the slots followed the split
+    jb = p.submit(droplet(engine, 6000, 63), 40)
+    ja = p.submit(a_sys(), 3)
+    p.advance(max_passes=8)                         # a is done on the device, its word not handled
+    assert p.state(ja) == engine.JOB_RUNNING
+    p.tune("split", 4)
+    p.run()
+    assert p.state(ja) == engine.JOB_DONE and p.state(jb) == engine.JOB_DONE
+    got = ({k: p.gather(ja, k) for k in ("x", "v", "f", "image", "e_pair", "virial")}, p.thermo(ja),
+           p.info(ja)["timestep"])
+    assert_same(got, alone, "job harvested after a split change")

@@ -28,6 +28,16 @@ constexpr int IMG_ZERO = 13;       // (0,0,0) image: (sx+1) + 3(sy+1) + 9(sz+1)
 constexpr int NBR_CHUNK = 8;
 constexpr int NBR_ALIGN = 32;
 
+// Compact-row counts of a step: MAXSPLIT slots of 64 per tile whatever the
+// split of the pass, so passes at different splits never write each other's
+// tiles' counts (a finished pooled system keeps the counts of its last pass
+// until its harvest pass reads them).
+constexpr int MAXSPLIT = 4;
+__host__ __device__ inline size_t ccount_at(int tile, int w, int lane)
+{
+    return ((size_t)tile * MAXSPLIT + w) * WAVE + lane;
+}
+
 __host__ __device__ inline size_t nbr_at(int i, int q, int maxnbr)
 {
     return (size_t)(i >> 6) * maxnbr * WAVE + (size_t)(q & ~(NBR_CHUNK - 1)) * WAVE +

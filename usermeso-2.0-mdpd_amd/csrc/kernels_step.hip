@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
     for (int off = 32; off > 0; off >>= 1) ncmax = max(ncmax, __shfl_xor(ncmax, off, 64));
     for (int t = 0; t < ncmax; ++t)
         if (t < nc) crow[(size_t)(t * SPLIT + w) * WAVE] = s_c[w][t * WAVE + lane];
-    P.ccount[((size_t)tile * SPLIT + w) * WAVE + lane] = nc;
+    P.ccount[ccount_at(tile, w, lane)] = nc;
     if (WITH_RHO) {
         s_sum[w][lane] = sum;
         __syncthreads();
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
         ti = __float_as_int(pi.w);
         mk = maskv[i];
         if (STYLE == STYLE_MDPD) rho_i = P.rho[i];
-        cnt = P.ccount[((size_t)tile * SPLIT + w) * WAVE + lane];
+        cnt = P.ccount[ccount_at(tile, w, lane)];
     }
     // arbitrary boundary: a mobile particle scales gamma / sigma of its pairs
     // with wall particles by its wall fraction (pair_mdpd_meso.cu:169-185)

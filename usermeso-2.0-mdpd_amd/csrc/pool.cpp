@@ -137,6 +137,7 @@ struct Job {
     std::vector<int> tag, type, mask, image;
     long long step_end = 0;
     int nbuild = 0;
+    long long pass_admit = 0;             // index of the job's first pass (its setup pass)
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
 
@@ -222,6 +223,8 @@ struct mdpd_pool {
     std::vector<hipEvent_t> prof_free;
     double prof_ms[PP_COUNT] = {0, 0, 0, 0};
     long long prof_passes = 0;
+    double prof_particles = 0;            // sum over sampled passes of the particles they evaluate
+    double particle_builds = 0;           // finished jobs: sum of particles x neighbor builds
 
     ~mdpd_pool()
     {
@@ -377,7 +380,7 @@ struct mdpd_pool {
         xhold.alloc(cap); frc.alloc(cap); vend.alloc(cap);
         rho.alloc(cap); vir.alloc((size_t)6 * cap); phi[0].alloc(cap); phi[1].alloc(cap);
         pcell.alloc(cap); pslot.alloc(cap); psub.alloc(cap); tmp.alloc(cap);
-        ncount.alloc(cap); ccount.alloc((size_t)4 * cap);
+        ncount.alloc(cap); ccount.alloc((size_t)MAXSPLIT * cap);
         nlist.alloc((size_t)cap * maxnbr);
         clist.alloc((size_t)cap * maxnbr);
         wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE); status.alloc(max_sys);
@@ -574,6 +577,7 @@ struct mdpd_pool {
         j.cell_base = cbase;
         j.state = JOB_RUNNING;
         j.t_admit = now_s();
+        j.pass_admit = passes;
         slot_job[s] = j.id;
         ++nrunning;
         const auto combo = std::make_pair(d.style, j.ntypes == 1 ? 1 : 0);
@@ -719,6 +723,23 @@ struct mdpd_pool {
         ++captures;
     }
 
+    // particles of the systems that pass number p evaluates: a job admitted
+    // before pass a runs passes a .. a + nsteps (its setup pass, its steps,
+    // the last half-kick).  Scales the sampled kernel times to bytes per
+    // launch.  (A system waiting for wider rows is skipped by the passes but
+    // counted here; its window then ends later than counted.)
+    double active_particles(long long p) const
+    {
+        double sum = 0;
+        for (int sl = 0; sl < max_sys; ++sl) {
+            const int jid = slot_job[sl];
+            if (!jid) continue;
+            const Job &j = *jobs.at(jid);
+            if (p >= j.pass_admit && p <= j.pass_admit + j.nsteps) sum += j.n;
+        }
+        return sum;
+    }
+
     void run_passes(int m)
     {
         if (m <= 0) return;
@@ -729,6 +750,7 @@ struct mdpd_pool {
                 for (auto &e : pe.e) e = timing_event();
                 launch_pass(split, &pe);
                 prof_open.push_back(pe);
+                prof_particles += active_particles(passes + k);
             }
         } else if (use_graph) {
             if (!graph_valid) capture();
@@ -935,6 +957,8 @@ struct mdpd_pool {
             j.t_done = now_s();
             --nharvest;
             particle_steps += (double)j.n * (double)j.nsteps;
+            j.nbuild = d.builds;
+            particle_builds += (double)j.n * (double)d.builds;
             finished.push_back(j.id);
             ++done;
         }
@@ -1186,6 +1210,7 @@ int mdpd_pool_tune(mdpd_pool *pool, const char *knob, int value)
             pool->prof_every = value;
             for (double &m : pool->prof_ms) m = 0;
             pool->prof_passes = 0;
+            pool->prof_particles = 0;
         } else if (k == "poll") {
             if (value < 1 || value > 4096) fail(MDPD_E_ARG, "poll must be 1..4096 passes");
             pool->passes_per_poll = value;
@@ -1212,6 +1237,8 @@ int mdpd_pool_stat(mdpd_pool *pool, const char *name, double *value)
         else if (k == "queued") *value = (double)pool->queue.size();
         else if (k == "used_slots") *value = (double)pool->parts.used * 256.0;
         else if (k == "prof_passes") *value = (double)pool->prof_passes;
+        else if (k == "prof_particles") *value = pool->prof_particles;
+        else if (k == "particle_builds") *value = pool->particle_builds;
         else if (k == "prof_rebuild_ms") *value = pool->prof_ms[mdpd_pool::PP_REBUILD];
         else if (k == "prof_first_ms") *value = pool->prof_ms[mdpd_pool::PP_FIRST];
         else if (k == "prof_force_ms") *value = pool->prof_ms[mdpd_pool::PP_FORCE];

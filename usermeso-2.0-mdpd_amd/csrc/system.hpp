@@ -1572,7 +1572,7 @@ struct mdpd_system {
         xhold.alloc(cap); frc.alloc(cap); vend.alloc(cap);
         rho.alloc(cap); vir.alloc((size_t)6 * cap); phi[0].alloc(cap); phi[1].alloc(cap);
         pcell.alloc(cap); pslot.alloc(cap); psub.alloc(cap); tmp.alloc(cap); key.alloc(2 * (size_t)cap);
-        ncount.alloc(cap); ccount.alloc((size_t)4 * cap);
+        ncount.alloc(cap); ccount.alloc((size_t)MAXSPLIT * cap);
         nlist.alloc((size_t)cap * maxnbr);
         clist.alloc((size_t)cap * maxnbr);
         wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE); status.alloc(1);
