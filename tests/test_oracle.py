@@ -377,6 +377,14 @@ def test_free_flight_and_parabola(orc):
     want = x + v * t + 0.5 * np.array(g) * t * t
     assert np.allclose(xo, want, atol=1e-12)
     assert np.allclose(vo, v + np.array(g) * t, atol=1e-12)
+    # groups (fix_nve_meso.cu:83-92, fix_addforce_meso.cu:88-92): both feel
+    # the body force of group 4; particle 1 is integrated (group 2) and
+    # follows the parabola, particle 2 is outside the integration group and
+    # stays where it is with its velocity unchanged
+    xo, vo, *_ = orc.run(x, v + [0.0, 0.0, 0.3], [1, 2], [0, 0], [1 | 2 | 4, 1 | 4], p, 100, nve_groupbit=2,
+                         addforce_groupbit=4, addforce=g, skin=0.3)
+    assert np.allclose(xo[0], x[0] + (v[0] + [0.0, 0.0, 0.3]) * t + 0.5 * np.array(g) * t * t, atol=1e-12)
+    assert np.array_equal(xo[1], x[1]) and np.array_equal(vo[1], [0.0, 0.0, 0.3])
 
 
 # ---------------------------------------------------------------- CPU DPD statistics
@@ -442,6 +450,34 @@ def test_wall_ratio_closed_forms(orc):
     r = [orc.wall_ratio(p, 1.0) for p in np.linspace(0, 0.46, 47)]
     assert np.all(np.diff(r) > 0)
     assert orc.wall_ratio(0.49, 1.0) == pytest.approx(top, rel=1e-14)
+
+
+def test_wall_scaled_pair_force_two_particles(orc):
+    # pair_mdpd_meso.cu:169-185: for a mobile i and a wall j, gamma scales by
+    # ratio(phi_i) and sigma by sqrt(ratio); a wall i with a mobile j and a
+    # mobile-mobile pair are not scaled.  Hand-evaluated with A = B = 0 on
+    # two particles on the x axis (the dissipative and random terms apart)
+    L, dt, g, sg, phi_i = 10.0, 0.01, 4.5, 3.0, 0.3
+    ratio = orc.wall_ratio(phi_i, 1.0)
+    sig = [0x01234567, 0x89ABCDEF]
+    xi = float(orc.gaussian([sig[0]], [sig[1]])[0])
+    r, dv = 0.6, 0.8
+    x = np.array([[5.0, 5.0, 5.0], [5.0 + r, 5.0, 5.0]], dtype=np.float32)
+    v = np.array([[dv, 0.0, 0.0], [0.0, 0.0, 0.0]], dtype=np.float32)
+    rr = float(np.float32(5.0 + r) - np.float32(5.0))
+    dvx = float(np.float32(dv))
+    wc, dx = 1.0 - rr, -rr
+    a = orc.arb(2, 4, pair=True, fix=False)
+    phi = np.array([phi_i, 0.0])
+    for gam, sgm, scale in ((g, 0.0, ratio), (0.0, sg, math.sqrt(ratio))):
+        p = orc.params((0, 0, 0), (L, L, L), dt, mass=(1.0, 1.0), ntypes=2, cut=1.0, cut_r=RD, A=0.0, B=0.0,
+                       gamma=gam, sigma=sgm)
+        base = (-gam * wc * wc * dx * dvx / rr + sgm * wc * xi / math.sqrt(dt)) / rr * dx
+        for mask, want in (([1 | 2, 1 | 4], base * scale),      # mobile i, wall j: scaled
+                           ([1 | 4, 1 | 2], base),              # wall i, mobile j: not scaled
+                           ([1 | 2, 1 | 2], base)):             # mobile pair: not scaled
+            f, _, _ = orc.force_mdpd_arb(x, v, np.zeros(2), sig, [0, 1], mask, phi, p, a)
+            assert f[0, 0] == pytest.approx(want, rel=1e-12), (gam, sgm, mask)
 
 
 def _arb_state(orc, x, types, L):
