@@ -175,6 +175,16 @@ int mdpd_tea_gaussian(int device, int n, const unsigned int *a, const unsigned i
 int mdpd_tea_premix(int device, int n, const unsigned int *a, const unsigned int *b, int rounds,
                     unsigned int *out);
 
+/* mdpd_tea_gaussian without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops).  Inputs and outputs are staged in a
+ * fixed device array that each GPU holds until the process ends
+ * (3 x MDPD_TEA_SCRATCH_MAX words, 12 MiB); n > MDPD_TEA_SCRATCH_MAX returns
+ * MDPD_E_OVERFLOW before any device work.  All calls on one GPU share that
+ * array: calls that overlap on the same GPU race. */
+#define MDPD_TEA_SCRATCH_MAX (1 << 20)
+int mdpd_tea_gaussian_noalloc(int device, int n, const unsigned int *a, const unsigned int *b, float *out);
+
 /* ---- pooled engine (batched and served runs) -------------------------------
  * Many independent systems ("jobs") packed into one set of device arrays and
  * advanced together, one launch sequence per pass for the whole pool.  The

@@ -6,6 +6,7 @@
 // lammps_gather_atoms / lammps_scatter_atoms).
 #include "system.hpp"
 
+#include <mutex>
 #include <new>
 
 // ---------------------------------------------------------------------------
@@ -345,6 +346,63 @@ static int stateless(int device, int n, const unsigned int *a, const unsigned in
 int mdpd_tea_gaussian(int device, int n, const unsigned int *a, const unsigned int *b, float *out)
 {
     return stateless(device, n, a, b, out, sizeof(float), 4, true);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Pair Gaussians without any allocation: a file-scope device array stages
+// the inputs and the outputs (MDPD_TEA_SCRATCH_MAX words each), for callers
+// that may not allocate device memory.  Each GPU holds its own copy of the
+// array until the process ends; host code uses the device address that
+// hipGetSymbolAddress returns for the current device, never the array's
+// name.  Every call on a GPU uses the same array: calls that overlap on one
+// GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) uint32_t scratch[3 * (size_t)MDPD_TEA_SCRATCH_MAX];
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratch_mu;
+uint32_t *scratch_dev[SCRATCH_DEVICES] = {nullptr};
+
+// device address of `scratch` on `device` (made current by the caller),
+// looked up on the first call for that device
+int scratch_on(int device, uint32_t **p)
+{
+    if (device < 0 || device >= SCRATCH_DEVICES) return MDPD_E_ARG;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[device]) {
+        uint32_t *ptr = nullptr;
+        if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return MDPD_E_HIP;
+        scratch_dev[device] = ptr;
+    }
+    *p = scratch_dev[device];
+    return MDPD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mdpd_tea_gaussian_noalloc(int device, int n, const unsigned int *a, const unsigned int *b, float *out)
+{
+    if (n < 0 || (n > 0 && (!a || !b || !out))) return MDPD_E_ARG;
+    if (n > MDPD_TEA_SCRATCH_MAX) return MDPD_E_OVERFLOW;
+    if (n == 0) return MDPD_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return MDPD_E_HIP;
+    if (hipSetDevice(device) != hipSuccess) return MDPD_E_HIP;
+    uint32_t *s = nullptr;
+    const int rc = scratch_on(device, &s);
+    if (rc != MDPD_OK) return rc;
+    uint32_t *da = s, *db = s + MDPD_TEA_SCRATCH_MAX;
+    float *dout = (float *)(s + 2 * (size_t)MDPD_TEA_SCRATCH_MAX);
+    const size_t bytes = (size_t)n * 4;
+    hipError_t e = hipMemcpy(da, a, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, b, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_pair_gaussian(da, db, dout, n, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? MDPD_OK : MDPD_E_HIP;
 }
 
 int mdpd_tea_premix(int device, int n, const unsigned int *a, const unsigned int *b, int rounds, unsigned int *out)
