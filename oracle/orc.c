@@ -298,9 +298,10 @@ int orc_neighbors_fp32(int n, const float *x, const double lo[3], const double h
                         if (j == i && code == 13) continue;
                         float xj = x[3 * j] + shx, yj = x[3 * j + 1] + shy, zj = x[3 * j + 2] + shz;
                         float dx = x[3 * i] - xj, dy = x[3 * i + 1] - yj, dz = x[3 * i + 2] - zj;
-                        float dxx = dx * dx, dyy = dy * dy, dzz = dz * dz;
-                        float s = dxx + dyy;
-                        float r2 = s + dzz;
+                        /* dx*dx + dy*dy + dz*dz as the reference's -O3 nvcc
+                         * build contracts it (neigh_build_meso.cu:86-89):
+                         * one rounded product, then two fused multiply-adds */
+                        float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                         if (r2 <= rc2) {
                             if (c < maxnbr) row[c] = j | (code << 27);
                             ++c;

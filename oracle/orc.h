@@ -89,8 +89,9 @@ void orc_wrap_fp32(int n, float *x, const double lo[3], const double hi[3]);
 /* ---------------- neighbor sets (FP32 test, bit-exact) ---------------- */
 
 /* Brute force over all j and the 27 periodic images: j is a neighbor of i iff
- * (j, image) != (i, 0) and the FP32 expression ((dx*dx+dy*dy)+dz*dz) <=
- * (float)(cut*cut), images shifted in FP32 by (float)(hi-lo);
+ * (j, image) != (i, 0) and the FP32 expression fma(dz, dz, fma(dy, dy, dx*dx))
+ * <= (float)(cut*cut) (the reference's -O3 build contracts dx*dx+dy*dy+dz*dz
+ * into that), images shifted in FP32 by (float)(hi-lo);
  * neigh_build_meso.cu:86-93.  Entries are encoded j | image << 27 with
  * image = (sx+1) + 3(sy+1) + 9(sz+1).  Writes counts[n]; list rows of width
  * maxnbr, each sorted ascending.  Returns the largest count (overflow if it

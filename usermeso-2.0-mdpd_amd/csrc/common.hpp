@@ -63,6 +63,23 @@ __device__ inline int xcd_logical(int b, int nlogical)
 }
 static inline int xcd_grid(int nlogical) { return ((nlogical + NXCD - 1) / NXCD) * NXCD; }
 
+// inclusive prefix sum over the 64 lanes of a wave (every lane active)
+__device__ inline int wave_incl_scan(int x, int lane)
+{
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// sum over the 64 lanes of a wave, in every lane (every lane active)
+__device__ inline int wave_sum(int x)
+{
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
 enum Style : int { STYLE_MDPD = 0, STYLE_DPD = 1 };
 
 // Lifecycle of a system in a pooled (batched / served) launch.  A job of K
@@ -137,6 +154,7 @@ struct PoolDev {
     int *tag[2], *mask[2], *origin[2], *image[2];
     float4 *xhold;       // positions at the last build
     float *rho;
+    float4 *pr;          // x, y, z and rho of the current step (written by the mDPD first pass)
     float *phi[2];       // wall fraction of the step (flips with tag / mask at re-sort)
     float4 *frc;         // fx, fy, fz, e_pair   (output steps only)
     float *vir;          // 6 * cap, SoA blocks  (output steps only)

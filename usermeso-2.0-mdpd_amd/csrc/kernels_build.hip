@@ -110,15 +110,6 @@ __global__ void __launch_bounds__(256) k_keys(PoolDev P)
 // --------------------------------------------------------------------------
 constexpr int SCAN_BLK = 256, SCAN_PER = 16, SCAN_CH = SCAN_BLK * SCAN_PER;
 
-__device__ inline int wave_incl_scan(int x, int lane)
-{
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    return x;
-}
-
 // exclusive scan of one value per thread over the workgroup; returns the
 // workgroup total through *total
 __device__ inline int block_excl_scan(int x, int *wsum, int *total)
@@ -300,9 +291,11 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 // K5: neighbor rows, one wave per cell.  A centre's row lists every
 // (j, image) of the 27 stencil cells of its cell -- stencil cell k has offset
 // (k%3-1, k/3%3-1, k/9-1), periodic images shift by +-L -- in stencil order,
-// then j ascending, with (float) r^2 <= (float)(rc+skin)^2 evaluated without
-// contraction (neigh_build_meso.cu:86-93), itself excluded.  The set is the
-// reference's (neigh_build_meso.cu:58-118).
+// then j ascending, with (float) r^2 <= (float)(rc+skin)^2, itself excluded.
+// r^2 = fma(dz, dz, fma(dy, dy, dx dx)): the reference's
+// `dx * dx + dy * dy + dz * dz` (neigh_build_meso.cu:86-93) as its -O3 build
+// (nvcc, floating-point contraction on by default) evaluates it.  The set is
+// the reference's (neigh_build_meso.cu:58-118).
 //
 // All centres of a cell share the stencil, so the wave walks the stencil's
 // candidates 64 at a time, one per lane (coalesced loads, image shift applied
@@ -408,7 +401,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8, 8)
                 for (int m = 0; m < NL_ITER; ++m) {
                     if (m >= nm) break;
                     const float dx = pcx - qx[m], dy = pcy - qy[m], dz = pcz - qz[m];
-                    const float r2 = (dx * dx + dy * dy) + dz * dz;
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                     const uint64_t mask = __builtin_amdgcn_ballot_w64(r2 <= rc2);
                     if (mask) {
                         if (__builtin_amdgcn_inverse_ballot_w64(mask))

@@ -212,8 +212,15 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
             for (int k = 1; k < SPLIT; ++k) tot += s_sum[k][lane];
             const int mk = P.mask[d->sela][i];
             const int ti = __float_as_int(pi.w);
-            P.rho[i] = (mk & d->rho_bit) ? P.mass[d->mass_base + ti] * d->rho_factor * tot : 0.f;
+            const float r = (mk & d->rho_bit) ? P.mass[d->mass_base + ti] * d->rho_factor * tot : 0.f;
+            P.rho[i] = r;
+            // the force pass of a one-type system gathers a neighbor's
+            // position and density with one 16-byte load
+            P.pr[i] = make_float4(pi.x, pi.y, pi.z, r);
         }
+    } else if (w == 0 && valid && d->style == STYLE_MDPD) {
+        // mDPD without the density fix: zero density (P.rho keeps its zeros)
+        P.pr[i] = make_float4(pi.x, pi.y, pi.z, 0.f);
     }
 }
 
@@ -346,6 +353,12 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
     constexpr bool STEP = MODE != MODE_SETUP && MODE != MODE_HARVEST;
     constexpr int NA = Acc<EV>::N;
     __shared__ float s_acc[SPLIT > 1 ? SPLIT - 1 : 1][NA][WAVE];
+    // lane-balanced pair loop (see below): one wave per tile, one type
+    constexpr bool BAL = SPLIT == 1 && UNI;
+    constexpr int TAIL_CAP = BAL ? 192 : 1;
+    __shared__ float4 s_pi[BAL ? WAVE : 1], s_vi[BAL ? WAVE : 1];
+    __shared__ float s_ri[BAL ? WAVE : 1];
+    __shared__ float s_tail[TAIL_CAP * NA];
     const int lt = xcd_logical(blockIdx.x, ntiles);
     if (lt >= ntiles) return;
     const int tile = tile0 + lt;
@@ -398,55 +411,131 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
     const int maxnbr = P.maxnbr;
     const int *crow = P.clist + (size_t)tile * maxnbr * WAVE + lane;
     Acc<EV> a;
-    int t = 0;
-    if (UNI) {
-        // single type: one coefficient set, four pairs' gathers in flight
-        for (; t + 4 <= cnt; t += 4) {
-            uint32_t e[4], j[4];
-            float4 q[4], vv[4];
-            float rj[4];
+    // pairs [t, end) of this lane's compact row, four gathers in flight (one
+    // type) or two
+    // neighbor j's position (q.w: its type bits, used by several types) and
+    // density; one type of mDPD takes both from the first pass's x, y, z,
+    // rho record with one 16-byte load
+    auto nbr = [&](uint32_t j, float4 &q, float &rj) {
+        if (UNI && STYLE == STYLE_MDPD) {
+            q = P.pr[j];
+            rj = q.w;
+        } else {
+            q = pos[j];
+            rj = STYLE == STYLE_MDPD ? P.rho[j] : 0.f;
+        }
+    };
+    auto walk = [&](int t, int end) {
+        if (UNI) {
+            for (; t + 4 <= end; t += 4) {
+                uint32_t e[4], j[4];
+                float4 q[4], vv[4];
+                float rj[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                e[u] = (uint32_t)crow[(size_t)((t + u) * SPLIT + w) * WAVE];
-                j[u] = e[u] & IDX_MASK;
-            }
+                for (int u = 0; u < 4; ++u) {
+                    e[u] = (uint32_t)crow[(size_t)((t + u) * SPLIT + w) * WAVE];
+                    j[u] = e[u] & IDX_MASK;
+                }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                q[u] = pos[j[u]];
-                vv[u] = vel[j[u]];
-                rj[u] = STYLE == STYLE_MDPD ? P.rho[j[u]] : 0.f;
-            }
+                for (int u = 0; u < 4; ++u) {
+                    nbr(j[u], q[u], rj[u]);
+                    vv[u] = vel[j[u]];
+                }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                PairC c = cu;
-                wall_scale(c, j[u]);
-                pair_term<STYLE, EV>(a, pi, vi, rho_i, q[u], vv[u], rj[u], e[u] >> IMG_SHIFT, c, dtis, d);
+                for (int u = 0; u < 4; ++u) {
+                    PairC c = cu;
+                    wall_scale(c, j[u]);
+                    pair_term<STYLE, EV>(a, pi, vi, rho_i, q[u], vv[u], rj[u], e[u] >> IMG_SHIFT, c, dtis, d);
+                }
             }
         }
-    }
-    for (; t + 2 <= cnt; t += 2) {
-        const uint32_t e0 = (uint32_t)crow[(size_t)(t * SPLIT + w) * WAVE];
-        const uint32_t e1 = (uint32_t)crow[(size_t)((t + 1) * SPLIT + w) * WAVE];
-        const uint32_t j0 = e0 & IDX_MASK, j1 = e1 & IDX_MASK;
-        const float4 q0 = pos[j0], q1 = pos[j1];
-        const float4 w0 = vel[j0], w1 = vel[j1];
-        const float r0 = STYLE == STYLE_MDPD ? P.rho[j0] : 0.f;
-        const float r1 = STYLE == STYLE_MDPD ? P.rho[j1] : 0.f;
-        PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
-        PairC c1 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q1.w));
-        wall_scale(c0, j0);
-        wall_scale(c1, j1);
-        pair_term<STYLE, EV>(a, pi, vi, rho_i, q0, w0, r0, e0 >> IMG_SHIFT, c0, dtis, d);
-        pair_term<STYLE, EV>(a, pi, vi, rho_i, q1, w1, r1, e1 >> IMG_SHIFT, c1, dtis, d);
-    }
-    if (t < cnt) {
-        const uint32_t e0 = (uint32_t)crow[(size_t)(t * SPLIT + w) * WAVE];
-        const uint32_t j0 = e0 & IDX_MASK;
-        const float4 q0 = pos[j0];
-        const float r0 = STYLE == STYLE_MDPD ? P.rho[j0] : 0.f;
-        PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
-        wall_scale(c0, j0);
-        pair_term<STYLE, EV>(a, pi, vi, rho_i, q0, vel[j0], r0, e0 >> IMG_SHIFT, c0, dtis, d);
+        for (; t + 2 <= end; t += 2) {
+            const uint32_t e0 = (uint32_t)crow[(size_t)(t * SPLIT + w) * WAVE];
+            const uint32_t e1 = (uint32_t)crow[(size_t)((t + 1) * SPLIT + w) * WAVE];
+            const uint32_t j0 = e0 & IDX_MASK, j1 = e1 & IDX_MASK;
+            float4 q0, q1;
+            float r0, r1;
+            nbr(j0, q0, r0);
+            nbr(j1, q1, r1);
+            const float4 w0 = vel[j0], w1 = vel[j1];
+            PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
+            PairC c1 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q1.w));
+            wall_scale(c0, j0);
+            wall_scale(c1, j1);
+            pair_term<STYLE, EV>(a, pi, vi, rho_i, q0, w0, r0, e0 >> IMG_SHIFT, c0, dtis, d);
+            pair_term<STYLE, EV>(a, pi, vi, rho_i, q1, w1, r1, e1 >> IMG_SHIFT, c1, dtis, d);
+        }
+        if (t < end) {
+            const uint32_t e0 = (uint32_t)crow[(size_t)(t * SPLIT + w) * WAVE];
+            const uint32_t j0 = e0 & IDX_MASK;
+            float4 q0;
+            float r0;
+            nbr(j0, q0, r0);
+            PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
+            wall_scale(c0, j0);
+            pair_term<STYLE, EV>(a, pi, vi, rho_i, q0, vel[j0], r0, e0 >> IMG_SHIFT, c0, dtis, d);
+        }
+    };
+    // Lane balance (one wave per tile, one type, no wall scaling): a wave
+    // runs as long as its longest row, ~1.5x its mean row.  The first M
+    // entries of every row (M: the mean row, raised until the rest fits
+    // TAIL_CAP) are walked by their own lane; the entries past M, the tails,
+    // laid out one after another in particle order, are dealt to all 64
+    // lanes.  A tail pair's terms go to the LDS slot of its position, and
+    // each particle adds its slots, in row order, after its own M entries.
+    bool balanced = false;
+    if constexpr (BAL) balanced = !(STYLE == STYLE_MDPD && d->pair_arb);
+    if (balanced) {
+        const int tot = wave_sum(cnt);
+        int M = tot >> 6;
+        int tail = wave_sum(max(cnt - M, 0));
+        for (int it = 0; tail > TAIL_CAP && it < 4; ++it) {
+            M += (tail - TAIL_CAP + WAVE - 1) / WAVE;
+            tail = wave_sum(max(cnt - M, 0));
+        }
+        if (tail > TAIL_CAP) {          // rows too uneven: the longest row sets M
+            M = cnt;
+            for (int off = 32; off > 0; off >>= 1) M = max(M, __shfl_xor(M, off, 64));
+            tail = 0;
+        }
+        walk(0, min(cnt, M));
+        if (tail > 0) {
+            const int sur = max(cnt - M, 0);
+            const int first = wave_incl_scan(sur, lane) - sur;     // this particle's first tail slot
+            s_pi[lane] = pi;
+            s_vi[lane] = vi;
+            s_ri[lane] = rho_i;
+            __syncthreads();
+            const int *ctile = P.clist + (size_t)tile * maxnbr * WAVE;
+            for (int f0 = 0; f0 < tail; f0 += WAVE) {
+                const int f = f0 + lane;
+                // owner: the last particle whose first slot is <= f (the
+                // particles before it with no tail share its first slot)
+                int p = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1)
+                    if (__shfl(first, p + step) <= f) p += step;
+                const int t = M + f - __shfl(first, p);
+                const bool act = f < tail;
+                // a lane past the last tail evaluates its own particle with
+                // itself (r = 0: no force) and stores nothing
+                const uint32_t e = act ? (uint32_t)ctile[(size_t)t * WAVE + p]
+                                       : ((uint32_t)i | ((uint32_t)IMG_ZERO << IMG_SHIFT));
+                const uint32_t j = e & IDX_MASK;
+                float4 q;
+                float rj;
+                nbr(j, q, rj);
+                Acc<EV> b;
+                pair_term<STYLE, EV>(b, s_pi[p], s_vi[p], s_ri[p], q, vel[j], rj, e >> IMG_SHIFT, cu, dtis, d);
+                if (act)
+                    for (int k = 0; k < NA; ++k) s_tail[k * TAIL_CAP + f] = b.v[k];
+            }
+            __syncthreads();
+            for (int f = first; f < first + sur; ++f)
+                for (int k = 0; k < NA; ++k) a.v[k] += s_tail[k * TAIL_CAP + f];
+        }
+    } else {
+        walk(0, cnt);
     }
     if (SPLIT > 1) {
         if (w > 0)

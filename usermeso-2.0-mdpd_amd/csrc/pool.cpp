@@ -145,7 +145,7 @@ struct Job {
 // slot (rank, coordinates, system; count and start per sub-cell), and per
 // neighbor-row entry (nlist + clist); CELLS_PER_SLOT cell slots are kept per
 // particle slot
-constexpr double BYTES_PER_SLOT = 240.0;
+constexpr double BYTES_PER_SLOT = 256.0;
 constexpr double BYTES_PER_CELL = 12.0 + 8.0 * SUBCELLS;
 constexpr double BYTES_PER_ENTRY = 8.0;
 constexpr int CELLS_PER_SLOT = 2;
@@ -167,7 +167,7 @@ struct mdpd_pool {
     int cap = 0, nslots = 0, maxnbr = 0;
     int nact = 0;                         // launch extent: high-water slot of the resident systems
     int ncact = 0;                        // per-cell launch extent: high-water cell slot
-    DevBuf<float4> pos[2], vel[2], xhold, frc, vend;
+    DevBuf<float4> pos[2], vel[2], xhold, frc, vend, pr;
     DevBuf<int> tag[2], mask[2], origin[2], image[2];
     DevBuf<float> rho, vir, coef, minv, mmass, phi[2];
     DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys;
@@ -283,7 +283,7 @@ struct mdpd_pool {
             P.pos[b] = pos[b].p; P.vel[b] = vel[b].p;
             P.tag[b] = tag[b].p; P.mask[b] = mask[b].p; P.origin[b] = origin[b].p; P.image[b] = image[b].p;
         }
-        P.xhold = xhold.p; P.rho = rho.p; P.frc = frc.p; P.vir = vir.p; P.vend = vend.p;
+        P.xhold = xhold.p; P.rho = rho.p; P.pr = pr.p; P.frc = frc.p; P.vir = vir.p; P.vend = vend.p;
         P.phi[0] = phi[0].p; P.phi[1] = phi[1].p;
         P.pcell = pcell.p; P.pslot = pslot.p; P.psub = psub.p; P.tmp = tmp.p; P.key = nullptr;
         P.cell_count = cell_count.p; P.cell_start = cell_start.p; P.cell_xyz = cell_xyz.p;
@@ -378,7 +378,7 @@ struct mdpd_pool {
             tag[b].alloc(cap); mask[b].alloc(cap); origin[b].alloc(cap); image[b].alloc(cap);
         }
         xhold.alloc(cap); frc.alloc(cap); vend.alloc(cap);
-        rho.alloc(cap); vir.alloc((size_t)6 * cap); phi[0].alloc(cap); phi[1].alloc(cap);
+        rho.alloc(cap); pr.alloc(cap); vir.alloc((size_t)6 * cap); phi[0].alloc(cap); phi[1].alloc(cap);
         pcell.alloc(cap); pslot.alloc(cap); psub.alloc(cap); tmp.alloc(cap);
         ncount.alloc(cap); ccount.alloc((size_t)MAXSPLIT * cap);
         nlist.alloc((size_t)cap * maxnbr);

@@ -217,7 +217,7 @@ struct mdpd_system {
     int cap = 0, maxnbr = 0, ncell = 0, nslots = 0;
     long long nbuild = 0;
     SysDev hsys{};
-    DevBuf<float4> pos[2], vel[2], xhold, frc, vend;
+    DevBuf<float4> pos[2], vel[2], xhold, frc, vend, pr;
     DevBuf<int> tag[2], mask[2], origin[2], image[2];
     DevBuf<float> rho, vir, coef, minv, mmass, phi[2];
     DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys;
@@ -1377,7 +1377,7 @@ struct mdpd_system {
             P.pos[b] = pos[b].p; P.vel[b] = vel[b].p;
             P.tag[b] = tag[b].p; P.mask[b] = mask[b].p; P.origin[b] = origin[b].p; P.image[b] = image[b].p;
         }
-        P.xhold = xhold.p; P.rho = rho.p; P.frc = frc.p; P.vir = vir.p; P.vend = vend.p;
+        P.xhold = xhold.p; P.rho = rho.p; P.pr = pr.p; P.frc = frc.p; P.vir = vir.p; P.vend = vend.p;
         P.phi[0] = phi[0].p; P.phi[1] = phi[1].p;
         P.pcell = pcell.p; P.pslot = pslot.p; P.psub = psub.p; P.tmp = tmp.p; P.key = key.p;
         P.cell_count = cell_count.p; P.cell_start = cell_start.p; P.cell_xyz = cell_xyz.p;
@@ -1570,7 +1570,7 @@ struct mdpd_system {
             tag[b].alloc(cap); mask[b].alloc(cap); origin[b].alloc(cap); image[b].alloc(cap);
         }
         xhold.alloc(cap); frc.alloc(cap); vend.alloc(cap);
-        rho.alloc(cap); vir.alloc((size_t)6 * cap); phi[0].alloc(cap); phi[1].alloc(cap);
+        rho.alloc(cap); pr.alloc(cap); vir.alloc((size_t)6 * cap); phi[0].alloc(cap); phi[1].alloc(cap);
         pcell.alloc(cap); pslot.alloc(cap); psub.alloc(cap); tmp.alloc(cap); key.alloc(2 * (size_t)cap);
         ncount.alloc(cap); ccount.alloc((size_t)MAXSPLIT * cap);
         nlist.alloc((size_t)cap * maxnbr);
