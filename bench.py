@@ -62,7 +62,7 @@ B_REBUILD = 52 + 52 + 16 + 4 * NLIST + 4                     # 347 B per particl
 # HBM bytes per particle moved by the pooled force pass, measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (medians over the passes of 64
 # droplets, counters scaled by a 1 GiB copy: x2.0 fetch, x1.0 write)
-T_FORCE_POOL = 181.4 + 32.1
+T_FORCE_POOL = 176.0 + 32.1
 GB = float(1 << 30)
 
 SLABS = {
@@ -185,13 +185,16 @@ def system(box, x, v, seed, lucy=False):
 # ---------------------------------------------------------------------------
 # plain path: one handle per system
 # ---------------------------------------------------------------------------
-def plain_job(box, x, v, steps, seed, lucy=False, split=None, graph=True):
+PLAIN_GRAPH = [True]          # --no-graph: plain runs launch every step's kernels directly
+
+
+def plain_job(box, x, v, steps, seed, lucy=False, split=None):
     """Wall time of `run steps` after setup (the run returns after the
     stream has drained)."""
     s = system(box, x, v, seed, lucy)
     if split:
         s.tune("split", split)
-    if not graph:
+    if not PLAIN_GRAPH[0]:
         s.tune("graph", 0)
     s.setup()
     t0 = time.perf_counter()
@@ -585,9 +588,10 @@ def run_headline(a, legs):
         log(f"batched: {value:.3e} particle-steps/s over {dt:.2f} s")
         med = int(np.argsort(sizes)[len(sizes) // 2])
         box, x, v, seed = specs[med]
-        psplit, ptimes = plain_best_split(box, x, v, seed)
-        pv, pns, ps = (plain_sample(sizes, lambda i: specs[i], a.job_steps, a.plain_sample, psplit)
-                       if a.plain_sample > 0 else (None, [], 0.0))
+        psplit, ptimes, pv, pns, ps = None, {}, None, [], 0.0
+        if a.plain_sample > 0:
+            psplit, ptimes = plain_best_split(box, x, v, seed)
+            pv, pns, ps = plain_sample(sizes, lambda i: specs[i], a.job_steps, a.plain_sample, psplit)
         log(f"plain: {pv:.3e} particle-steps/s (split {psplit})" if pv else "plain: skipped")
         config.update({
             "timed_region_s": dt, "ms_per_step_definition": "timed region / --steps (the region is the whole "
@@ -806,7 +810,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work per baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-graph", action="store_true",
-                    help="batched and served pools: launch each pass's kernels directly instead of replaying a hipGraph")
+                    help="launch each pass's (pools) and step's (plain runs) kernels directly instead of replaying a "
+                         "hipGraph")
     ap.add_argument("--split", type=int, default=None, help="waves per tile (default: autotuned)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device with a plain copy kernel (a known-size "
@@ -816,6 +821,7 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+    PLAIN_GRAPH[0] = not a.no_graph
     if a.calibrate_copy_mib > 0:
         import torch
         src = torch.ones(a.calibrate_copy_mib << 18, dtype=torch.float32, device="cuda")

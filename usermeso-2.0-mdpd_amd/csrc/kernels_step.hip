@@ -94,7 +94,7 @@ __device__ inline float pair_rsq(const float4 &pi, const float4 &q, uint32_t cod
 template <bool WITH_RHO, int SPLIT>
 __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
 {
-    constexpr int STAGE = SPLIT == 1 ? 40 : 32 / SPLIT;   // LDS-staged compact entries per lane (measured best)
+    constexpr int STAGE = 32 / SPLIT;   // LDS-staged compact entries per lane (measured best)
     __shared__ int s_c[SPLIT][STAGE * WAVE];
     __shared__ float s_sum[SPLIT][WAVE];
     const int tile = xcd_logical(blockIdx.x, ntiles);
