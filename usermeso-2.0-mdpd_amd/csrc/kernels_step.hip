@@ -696,7 +696,16 @@ __global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance, int
     if (advance) {
         const int t0 = d->part_base / WAVE, nt = d->npad / WAVE;
         int any = 0;
-        for (int t = threadIdx.x; t < nt; t += blockDim.x) any |= P.moved[t0 + t];
+        // four tile flags in flight per thread (a 200k-particle system has ~3,000 tiles)
+        for (int t = threadIdx.x; t < nt; t += 4 * (int)blockDim.x) {
+            int f[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = t + u * (int)blockDim.x;
+                f[u] = k < nt ? P.moved[t0 + k] : 0;
+            }
+            any |= f[0] | f[1] | f[2] | f[3];
+        }
         if (any) s_any = 1;
         __syncthreads();
     }
