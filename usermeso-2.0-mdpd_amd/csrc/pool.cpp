@@ -151,6 +151,7 @@ constexpr double BYTES_PER_ENTRY = 8.0;
 constexpr int CELLS_PER_SLOT = 2;
 constexpr long long RETUNE_PASSES = 256;
 constexpr size_t ADMIT_LOOKAHEAD = 64;    // queued jobs considered per admission round
+constexpr int MAX_BATCH = 16;             // passes captured in one batch graph
 constexpr int MAX_BYPASS = 64;            // admissions past one waiting head job
 
 }  // namespace
@@ -190,7 +191,9 @@ struct mdpd_pool {
     double tuned_load = 0;                // resident particle slots when the split was last tuned
     double tune_ms[5] = {0, 0, 0, 0, 0};
     bool use_graph = true;
-    hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t gexec = nullptr;       // one pass
+    hipGraphExec_t gbatch = nullptr;      // batch_len passes, one launch
+    int batch_len = 0;
     bool graph_valid = false;
     int nsys_act = 64;                    // per-system launch extent (descriptor slots)
     int maxsub = 16384;                   // sub-cell slots of the largest resident system
@@ -230,6 +233,7 @@ struct mdpd_pool {
     {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (gbatch) (void)hipGraphExecDestroy(gbatch);
         for (auto &u : uploads) {
             (void)hipEventDestroy(u.first);
             (void)hipHostFree(u.second.first);
@@ -700,25 +704,38 @@ struct mdpd_pool {
         }
     }
 
-    void capture()
+    // npass passes of the current launch shape as one executable graph
+    hipGraphExec_t capture_passes(int npass)
     {
-        if (gexec) {
-            hip_check(hipGraphExecDestroy(gexec), "hipGraphExecDestroy");
-            gexec = nullptr;
-        }
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
-            launch_pass(split);
+            for (int k = 0; k < npass; ++k) launch_pass(split);
         } catch (...) {
             (void)hipStreamEndCapture(stream, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
         hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
-        const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        hipGraphExec_t x = nullptr;
+        const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         hip_check(e, "hipGraphInstantiate");
+        return x;
+    }
+
+    // the pass as a graph, and a batch of passes (one poll interval, at most
+    // MAX_BATCH) as another: one graph launch per batch instead of per pass
+    void capture()
+    {
+        for (hipGraphExec_t *x : {&gexec, &gbatch})
+            if (*x) {
+                hip_check(hipGraphExecDestroy(*x), "hipGraphExecDestroy");
+                *x = nullptr;
+            }
+        gexec = capture_passes(1);
+        batch_len = std::min(passes_per_poll, MAX_BATCH);
+        gbatch = batch_len > 1 ? capture_passes(batch_len) : nullptr;
         graph_valid = true;
         ++captures;
     }
@@ -754,7 +771,10 @@ struct mdpd_pool {
             }
         } else if (use_graph) {
             if (!graph_valid) capture();
-            for (int k = 0; k < m; ++k) hip_check(hipGraphLaunch(gexec, stream), "hipGraphLaunch");
+            int k = 0;
+            if (gbatch)
+                for (; k + batch_len <= m; k += batch_len) hip_check(hipGraphLaunch(gbatch, stream), "hipGraphLaunch");
+            for (; k < m; ++k) hip_check(hipGraphLaunch(gexec, stream), "hipGraphLaunch");
         } else {
             for (int k = 0; k < m; ++k) launch_pass(split);
         }
@@ -1214,6 +1234,7 @@ int mdpd_pool_tune(mdpd_pool *pool, const char *knob, int value)
         } else if (k == "poll") {
             if (value < 1 || value > 4096) fail(MDPD_E_ARG, "poll must be 1..4096 passes");
             pool->passes_per_poll = value;
+            pool->graph_valid = false;
         } else {
             fail(MDPD_E_ARG, "unknown tuning knob '%s'", knob);
         }
