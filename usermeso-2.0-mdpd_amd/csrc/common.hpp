@@ -170,7 +170,9 @@ struct PoolDev {
     int *cell_xyz;       // packed interior coords of the cell at this Morton rank
     int *cell_rank;      // Morton rank of linear cell id (per system, cell_base + lin)
     int *cell_sys;       // system of each cell slot
-    int *nlist;          // (cap/64) tiles * maxnbr * 64
+    int *cell_work;      // cell slots holding particles of the systems re-sorted this pass (k_scan_apply appends)
+    int *work_count;     // [0]: entries of cell_work (k_commit_rebuild clears it)
+    int *nlist;         // (cap/64) tiles * maxnbr * 64
     int *ncount;         // per particle
     int *clist;          // per step: entries of nlist within the pair cutoff, same layout
     int *ccount;         // per particle
