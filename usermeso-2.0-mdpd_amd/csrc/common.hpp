@@ -170,8 +170,10 @@ struct PoolDev {
     int *cell_xyz;       // packed interior coords of the cell at this Morton rank
     int *cell_rank;      // Morton rank of linear cell id (per system, cell_base + lin)
     int *cell_sys;       // system of each cell slot
-    int *cell_work;      // cell slots holding particles of the systems re-sorted this pass (k_scan_apply appends)
+    int4 *cell_work;     // pools: the cells holding particles of the systems re-sorted this pass, as
+                         // (system, packed coordinates, first particle, end) (k_scan_apply appends)
     int *work_count;     // [0]: entries of cell_work (k_commit_rebuild clears it)
+    int2 *cell_lse;      // pools: particle range [start, end) of each cell by linear cell id (cell_base + lin)
     int *nlist;         // (cap/64) tiles * maxnbr * 64
     int *ncount;         // per particle
     int *clist;          // per step: entries of nlist within the pair cutoff, same layout

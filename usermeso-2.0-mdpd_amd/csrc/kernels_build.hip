@@ -201,20 +201,36 @@ __global__ void __launch_bounds__(SCAN_BLK) k_scan_apply(PoolDev P)
         __syncthreads();
         int v[SCAN_PER], sum = 0;
         for (int k = 0; k < SCAN_PER; ++k) { v[k] = buf[t * SCAN_PER + k]; sum += v[k]; }
-        // the thread's SCAN_PER / SUBCELLS whole cells: the ones holding
-        // particles go to the neighbor-row worklist, in cell-slot order
-        // within the chunk (one atomic per workgroup and chunk: same-address
-        // atomics serialize); counts past the system's end are zero, so only
-        // its own cells are listed
+        static_assert(SCAN_PER % SUBCELLS == 0 && SCAN_CH % SUBCELLS == 0, "whole cells per thread");
+        constexpr int CPT = SCAN_PER / SUBCELLS;                 // cells per thread
+        int cst[CPT];                                            // first particle of each
+        int total;
+        int run = off + block_excl_scan(sum, wsum, &total);
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; ++k) {
+            if (k % SUBCELLS == 0) cst[k / SUBCELLS] = run;
+            buf[t * SCAN_PER + k] = run;
+            run += v[k];
+        }
+        // pools: every cell's particle range by linear cell id, and the cells
+        // holding particles appended to the neighbor-row worklist with their
+        // system, coordinates and range, in cell-slot order within the chunk
+        // (one atomic per workgroup and chunk: same-address atomics
+        // serialize); counts past the system's end are zero and not listed
         if (nlist_listed(P)) {
-            static_assert(SCAN_PER % SUBCELLS == 0 && SCAN_CH % SUBCELLS == 0, "whole cells per thread");
-            constexpr int CPT = SCAN_PER / SUBCELLS;
-            int full = 0;
+            const int cell0 = d->cell_base + (c0 + t * SCAN_PER) / SUBCELLS;
+            int xyz[CPT], full = 0;
 #pragma unroll
             for (int q = 0; q < CPT; ++q) {
+                const bool in = c0 + t * SCAN_PER + q * SUBCELLS < nc;
                 int m = 0;
 #pragma unroll
                 for (int u = 0; u < SUBCELLS; ++u) m += v[q * SUBCELLS + u];
+                xyz[q] = in ? P.cell_xyz[cell0 + q] : 0;
+                if (in) {
+                    const int lin = (xyz[q] & 1023) + d->nb[0] * (((xyz[q] >> 10) & 1023) + d->nb[1] * ((xyz[q] >> 20) & 1023));
+                    P.cell_lse[d->cell_base + lin] = make_int2(cst[q], cst[q] + m);
+                }
                 full |= (m > 0 ? 1 : 0) << q;
             }
             const int cnt = __popc(full);
@@ -230,14 +246,16 @@ __global__ void __launch_bounds__(SCAN_BLK) k_scan_apply(PoolDev P)
             __syncthreads();
             int o = s_wl[SCAN_BLK / 64] + incl - cnt;
             for (int k = 0; k < wv; ++k) o += s_wl[k];
-            const int cell0 = d->cell_base + (c0 + t * SCAN_PER) / SUBCELLS;
 #pragma unroll
-            for (int q = 0; q < CPT; ++q)
-                if (full >> q & 1) P.cell_work[o++] = cell0 + q;
+            for (int q = 0; q < CPT; ++q) {
+                if (full >> q & 1) {
+                    int m = 0;
+#pragma unroll
+                    for (int u = 0; u < SUBCELLS; ++u) m += v[q * SUBCELLS + u];
+                    P.cell_work[o++] = make_int4((int)blockIdx.y, xyz[q], cst[q], cst[q] + m);
+                }
+            }
         }
-        int total;
-        int run = off + block_excl_scan(sum, wsum, &total);
-        for (int k = 0; k < SCAN_PER; ++k) { buf[t * SCAN_PER + k] = run; run += v[k]; }
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < SCAN_PER; ++u) {
@@ -368,16 +386,28 @@ constexpr int NL_GRID = 65536;         // its waves (measured against 7,168 and 
                                        // waves balance uneven cells better; 65,536 cost 14 us
                                        // to dispatch when no cell is listed)
 
-__device__ __attribute__((always_inline)) void nlist_cell(const PoolDev &P, int cs, int relist, int *s_row)
+// LISTED: e = (system, packed coordinates, first particle, end) of a listed
+// cell, whose neighbor cells' ranges come from the linear table; otherwise
+// e.x = a cell slot, looked up through its system and the Morton rank table
+template <bool LISTED>
+__device__ __attribute__((always_inline)) void nlist_cell(const PoolDev &P, int4 e, int relist, int *s_row)
 {
-    const int s = P.cell_sys[cs];
-    if (s < 0) return;
+    int s, xyz, c0, c1, cs = e.x;
+    if (LISTED) {
+        s = e.x; xyz = e.y; c0 = e.z; c1 = e.w;
+    } else {
+        s = P.cell_sys[cs];
+        if (s < 0) return;
+    }
     SysDev *d = P.sys + s;
-    if (!d->active || (relist ? d->stale != 0 : d->stale == 0)) return;
     const int cb = d->cell_base;
-    if (cs < cb || cs >= cb + d->ncell) return;           // the end slot holds no cell
-    const int c0 = P.cell_start[cs * SUBCELLS], c1 = P.cell_start[(cs + 1) * SUBCELLS];
-    if (c0 >= c1) return;
+    if (!LISTED) {
+        if (!d->active || (relist ? d->stale != 0 : d->stale == 0)) return;
+        if (cs < cb || cs >= cb + d->ncell) return;       // the end slot holds no cell
+        c0 = P.cell_start[cs * SUBCELLS];
+        c1 = P.cell_start[(cs + 1) * SUBCELLS];
+        if (c0 >= c1) return;
+    }
     const int lane = threadIdx.x;
     const float4 *pos = P.xhold;
     // stencil table: lane k < 27 holds cell k's first particle, its length,
@@ -386,7 +416,7 @@ __device__ __attribute__((always_inline)) void nlist_cell(const PoolDev &P, int 
     uint32_t kcode = IMG_ZERO;
     float ksx = 0.f, ksy = 0.f, ksz = 0.f;
     if (lane < 27) {
-        const int xyz = P.cell_xyz[cs];
+        if (!LISTED) xyz = P.cell_xyz[cs];
         const int cx = xyz & 1023, cy = (xyz >> 10) & 1023, cz = (xyz >> 20) & 1023;
         const int nb0 = d->nb[0], nb1 = d->nb[1], nb2 = d->nb[2];
         int nx = cx + lane % 3 - 1, ny = cy + lane / 3 % 3 - 1, nz = cz + lane / 9 - 1;
@@ -394,9 +424,16 @@ __device__ __attribute__((always_inline)) void nlist_cell(const PoolDev &P, int 
         if (nx < 0) { nx += nb0; sx = -1; } else if (nx >= nb0) { nx -= nb0; sx = 1; }
         if (ny < 0) { ny += nb1; sy = -1; } else if (ny >= nb1) { ny -= nb1; sy = 1; }
         if (nz < 0) { nz += nb2; sz = -1; } else if (nz >= nb2) { nz -= nb2; sz = 1; }
-        const int rank = P.cell_rank[cb + nx + nb0 * (ny + nb1 * nz)];
-        kst = P.cell_start[(cb + rank) * SUBCELLS];
-        klen = P.cell_start[(cb + rank + 1) * SUBCELLS] - kst;
+        const int lin = nx + nb0 * (ny + nb1 * nz);
+        if (LISTED) {
+            const int2 r = P.cell_lse[cb + lin];
+            kst = r.x;
+            klen = r.y - r.x;
+        } else {
+            const int rank = P.cell_rank[cb + lin];
+            kst = P.cell_start[(cb + rank) * SUBCELLS];
+            klen = P.cell_start[(cb + rank + 1) * SUBCELLS] - kst;
+        }
         kcode = (uint32_t)((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
         ksx = (float)sx * d->L[0];
         ksy = (float)sy * d->L[1];
@@ -495,7 +532,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8, 8)
 {
     __shared__ int s_row[NL_ITER * WAVE];                   // one centre's entries of one batch
     const int cs = xcd_logical(blockIdx.x, ncells);        // cell slot (Morton rank order)
-    if (cs < ncells) nlist_cell(P, cs, relist, s_row);
+    if (cs < ncells) nlist_cell<false>(P, make_int4(cs, 0, 0, 0), relist, s_row);
 }
 
 // the worklist walk
@@ -507,7 +544,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(NL_WA
     const int x = blockIdx.x % NXCD, per = gridDim.x / NXCD;
     const int hi = (int)((long long)n * (x + 1) / NXCD);
     for (int w = (int)((long long)n * x / NXCD) + blockIdx.x / NXCD; w < hi; w += per)
-        nlist_cell(P, P.cell_work[w], 0, s_row);
+        nlist_cell<true>(P, P.cell_work[w], 0, s_row);
 }
 
 // overflow flags of the sorted systems cleared before their rows are rebuilt

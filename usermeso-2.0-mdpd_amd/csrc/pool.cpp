@@ -142,11 +142,11 @@ struct Job {
 };
 
 // bytes of device memory per particle slot (state and scratch), per cell
-// slot (rank, coordinates, system, worklist entry; count and start per
-// sub-cell), and per neighbor-row entry (nlist + clist); CELLS_PER_SLOT cell
-// slots are kept per particle slot
+// slot (rank, coordinates, system; worklist entry, linear range; count and
+// start per sub-cell), and per neighbor-row entry (nlist + clist);
+// CELLS_PER_SLOT cell slots are kept per particle slot
 constexpr double BYTES_PER_SLOT = 256.0;
-constexpr double BYTES_PER_CELL = 16.0 + 8.0 * SUBCELLS;
+constexpr double BYTES_PER_CELL = 12.0 + 16.0 + 8.0 + 8.0 * SUBCELLS;
 constexpr double BYTES_PER_ENTRY = 8.0;
 constexpr int CELLS_PER_SLOT = 2;
 constexpr long long RETUNE_PASSES = 256;
@@ -171,7 +171,9 @@ struct mdpd_pool {
     DevBuf<float4> pos[2], vel[2], xhold, frc, vend, pr;
     DevBuf<int> tag[2], mask[2], origin[2], image[2];
     DevBuf<float> rho, vir, coef, minv, mmass, phi[2];
-    DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys, cell_work;
+    DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys, work_count;
+    DevBuf<int4> cell_work;
+    DevBuf<int2> cell_lse;
     DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl;
     DevBuf<SysDev> dsys;
     RangeAlloc parts, cells;              // parts in units of 256 slots
@@ -292,7 +294,7 @@ struct mdpd_pool {
         P.pcell = pcell.p; P.pslot = pslot.p; P.psub = psub.p; P.tmp = tmp.p; P.key = nullptr;
         P.cell_count = cell_count.p; P.cell_start = cell_start.p; P.cell_xyz = cell_xyz.p;
         P.cell_rank = cell_rank.p; P.cell_sys = cell_sys.p;
-        P.work_count = cell_work.p; P.cell_work = cell_work.p + 64;
+        P.work_count = work_count.p; P.cell_work = cell_work.p; P.cell_lse = cell_lse.p;
         P.nlist = nlist.p; P.ncount = ncount.p; P.wave_sys = wave_sys.p;
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
         P.cl = cl.p;
@@ -393,13 +395,13 @@ struct mdpd_pool {
         cell_count.alloc((size_t)nslots * SUBCELLS); cell_start.alloc((size_t)nslots * SUBCELLS);
         cell_xyz.alloc(nslots);
         cell_rank.alloc(nslots); cell_sys.alloc(nslots);
-        cell_work.alloc((size_t)nslots + 64);     // [0]: count, then the cell slots
+        cell_work.alloc(nslots); cell_lse.alloc(nslots); work_count.alloc(64);
         coef.alloc((size_t)max_sys * MAXTYPES * MAXTYPES * NCOEF);
         minv.alloc((size_t)max_sys * MAXTYPES); mmass.alloc((size_t)max_sys * MAXTYPES);
         dsys.alloc(max_sys);
         hip_check(hipMemsetAsync(cell_count.p, 0, sizeof(int) * (size_t)nslots * SUBCELLS, stream), "memset");
         hip_check(hipMemsetAsync(cell_sys.p, 0xFF, sizeof(int) * nslots, stream), "memset");
-        hip_check(hipMemsetAsync(cell_work.p, 0, sizeof(int), stream), "memset");
+        hip_check(hipMemsetAsync(work_count.p, 0, sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(wave_sys.p, 0xFF, sizeof(int) * (cap / WAVE), stream), "memset");
         hip_check(hipMemsetAsync(status.p, 0, sizeof(int) * max_sys, stream), "memset");
         hip_check(hipMemsetAsync(dsys.p, 0, sizeof(SysDev) * max_sys, stream), "memset");

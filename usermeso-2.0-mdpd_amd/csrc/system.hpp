@@ -220,7 +220,9 @@ struct mdpd_system {
     DevBuf<float4> pos[2], vel[2], xhold, frc, vend, pr;
     DevBuf<int> tag[2], mask[2], origin[2], image[2];
     DevBuf<float> rho, vir, coef, minv, mmass, phi[2];
-    DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys, cell_work;
+    DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys, work_count;
+    DevBuf<int4> cell_work;
+    DevBuf<int2> cell_lse;
     DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl;
     DevBuf<unsigned long long> key;
     DevBuf<SysDev> dsys;
@@ -1382,7 +1384,7 @@ struct mdpd_system {
         P.pcell = pcell.p; P.pslot = pslot.p; P.psub = psub.p; P.tmp = tmp.p; P.key = key.p;
         P.cell_count = cell_count.p; P.cell_start = cell_start.p; P.cell_xyz = cell_xyz.p;
         P.cell_rank = cell_rank.p; P.cell_sys = cell_sys.p;
-        P.work_count = cell_work.p; P.cell_work = cell_work.p + 64;
+        P.work_count = work_count.p; P.cell_work = cell_work.p; P.cell_lse = cell_lse.p;
         P.nlist = nlist.p; P.ncount = ncount.p; P.wave_sys = wave_sys.p;
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
         P.cl = cl.p;
@@ -1581,7 +1583,7 @@ struct mdpd_system {
         cell_count.alloc((size_t)nslots * SUBCELLS); cell_start.alloc((size_t)nslots * SUBCELLS);
         cell_xyz.alloc(nslots);
         cell_rank.alloc(nslots); cell_sys.alloc(nslots);
-        cell_work.alloc((size_t)nslots + 64);     // [0]: count, then the cell slots
+        cell_work.alloc(nslots); cell_lse.alloc(nslots); work_count.alloc(64);
         coef.alloc(ds.coef.size()); minv.alloc(ntypes); mmass.alloc(ntypes); dsys.alloc(1);
 
         // ---- upload ----
@@ -1606,7 +1608,7 @@ struct mdpd_system {
         up(minv.p, ds.minv.data(), ntypes * sizeof(float));
         up(mmass.p, ds.mass.data(), ntypes * sizeof(float));
         hip_check(hipMemsetAsync(cell_count.p, 0, (size_t)nslots * SUBCELLS * sizeof(int), stream), "memset");
-        hip_check(hipMemsetAsync(cell_work.p, 0, sizeof(int), stream), "memset");
+        hip_check(hipMemsetAsync(work_count.p, 0, sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(ncount.p, 0, cap * sizeof(int), stream), "memset");
         hip_check(hipMemsetAsync(rho.p, 0, cap * sizeof(float), stream), "memset");
         std::vector<float> hph(cap, 0.f);   // per-atom state of fix aboundary/rho/meso
