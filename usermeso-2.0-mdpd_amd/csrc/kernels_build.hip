@@ -110,7 +110,9 @@ __global__ void __launch_bounds__(256) k_keys(PoolDev P)
 //   k_scan_sums   per chunk: its total, parked in the chunk's first start slot
 //   k_scan_chunks per system: exclusive scan of the chunk totals in place,
 //                 and the end entry
-//   k_scan_apply  per chunk: exclusive scan of its counts from its offset
+//   k_scan_apply  per chunk: exclusive scan of its counts from its offset;
+//                 in a pool also each cell's particle range by linear cell
+//                 id and the worklist of occupied cells for the row build
 // Grid (G, nsys): workgroup g of a system takes chunks g, g + G, ...
 // --------------------------------------------------------------------------
 constexpr int SCAN_BLK = 256, SCAN_PER = 16, SCAN_CH = SCAN_BLK * SCAN_PER;
@@ -369,8 +371,10 @@ __global__ void __launch_bounds__(256) k_sortgather(PoolDev P)
 // the rebuild trigger (displacement from xhold) keeps its guarantee.
 // relist selects the systems not flagged stale, otherwise the stale ones.
 //
-// A rebuild walks the worklist k_scan_apply made: the cells holding
-// particles of the systems re-sorted this pass.  A pool's cell slots are
+// A pool's rebuild walks the worklist k_scan_apply made: the cells holding
+// particles of the systems re-sorted this pass, each with its system,
+// coordinates and particle range; a listed cell reads its 27 neighbor ranges
+// from the linear range table.  A pool's cell slots are
 // mostly empty (a droplet fills a quarter of its box) or belong to systems
 // not re-sorted, and dispatching one workgroup per cell slot costs ~0.2 ns
 // each whether it works or exits (136 us per pass for 650,000 slots), so a
