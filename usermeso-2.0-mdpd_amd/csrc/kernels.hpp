@@ -44,8 +44,14 @@ hipError_t launch_force_range(const PoolDev &P, int style, int mode, bool uni, i
 // wall branch of aboundary/rho/meso (wall fraction, bounce-back, displacement
 // flags) on the buffer a pass just wrote; pooled: only systems that integrated
 hipError_t launch_aboundary(const PoolDev &P, int pooled, hipStream_t st);
-// pooled engine, finished system s: current data into buffer 0 (sel = sela = 0)
-hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, hipStream_t st);
+// pooled engine, finished system s (npad slots from part_base): its results
+// packed into dst (room bytes, at least harvest_bytes(npad)) in this layout:
+//   float4 x[npad] | float4 v[npad] | float4 f[npad] (w: pair energy) |
+//   int tag[npad], mask[npad], origin[npad], image[npad] | float rho[npad] |
+//   float virial[6][npad] | SysDev
+constexpr size_t HARVEST_SLOT_BYTES = 3 * 16 + 4 * 4 + 4 + 6 * 4;
+inline size_t harvest_bytes(size_t npad) { return npad * HARVEST_SLOT_BYTES + sizeof(SysDev); }
+hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, void *dst, size_t room, hipStream_t st);
 
 // observables (accumulate into device histograms)
 hipError_t launch_rdf(const PoolDev &P, unsigned *hist, int nbin, float rc, int gi, int gj, hipStream_t st);

@@ -740,27 +740,36 @@ __global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance, int
 }
 
 // --------------------------------------------------------------------------
-// Finished pooled system: its current particle data into buffer 0, so the
-// host copies one contiguous range per array.  Runs after the HARVEST force
-// pass, which reads the other buffer.
+// Finished pooled system: its current state, the last force evaluation and
+// its descriptor packed into one block at dst (x, v, f | tag, mask, origin,
+// image | rho, virial | SysDev, each array npad long), so the host copies one
+// range.  Runs after the HARVEST force pass, which reads the other buffer.
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_harvest(PoolDev P, int s, int pb, int npad)
+__global__ void __launch_bounds__(256) k_harvest(PoolDev P, int s, int pb, int npad, char *dst)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= npad) return;
     const SysDev *d = P.sys + s;
-    const int i = pb + k;
-    if (d->sel) {
-        P.pos[0][i] = P.pos[1][i];
-        P.vel[0][i] = P.vel[1][i];
+    if (blockIdx.x == 0) {
+        const int *src = (const int *)d;
+        int *o = (int *)(dst + (size_t)npad * HARVEST_SLOT_BYTES);
+        for (int w = threadIdx.x; w < (int)(sizeof(SysDev) / 4); w += blockDim.x) o[w] = src[w];
     }
-    if (d->sela) {
-        P.tag[0][i] = P.tag[1][i];
-        P.mask[0][i] = P.mask[1][i];
-        P.origin[0][i] = P.origin[1][i];
-        P.image[0][i] = P.image[1][i];
-        P.phi[0][i] = P.phi[1][i];
-    }
+    if (k >= npad) return;
+    const int i = pb + k, cur = d->sel, cura = d->sela;
+    const size_t n = npad;
+    float4 *o4 = (float4 *)dst;
+    o4[k] = P.pos[cur][i];
+    o4[n + k] = P.vel[cur][i];
+    o4[2 * n + k] = P.frc[i];
+    int *oi = (int *)(o4 + 3 * n);
+    oi[k] = P.tag[cura][i];
+    oi[n + k] = P.mask[cura][i];
+    oi[2 * n + k] = P.origin[cura][i];
+    oi[3 * n + k] = P.image[cura][i];
+    float *of = (float *)(oi + 4 * n);
+    of[k] = P.rho[i];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) of[(c + 1) * n + k] = P.vir[(size_t)c * P.cap + i];
 }
 
 // --------------------------------------------------------------------------
@@ -822,11 +831,12 @@ hipError_t launch_aboundary(const PoolDev &P, int pooled, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, hipStream_t st)
+hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, void *dst, size_t room, hipStream_t st)
 {
     if (npad <= 0) return hipSuccess;
-    if (s < 0 || s >= P.nsys || part_base < 0 || part_base + npad > P.cap) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_harvest, dim3((npad + 255) / 256), dim3(256), 0, st, P, s, part_base, npad);
+    if (s < 0 || s >= P.nsys || part_base < 0 || part_base + npad > P.cap || !dst || harvest_bytes(npad) > room)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_harvest, dim3((npad + 255) / 256), dim3(256), 0, st, P, s, part_base, npad, (char *)dst);
     return hipGetLastError();
 }
 

@@ -127,19 +127,47 @@ struct Job {
     // placement while admitted
     int slot = -1;
     long long part_base = -1, cell_base = -1;
-    // harvest in flight
+    // harvest: the packed results (launch_harvest's layout, slot order) in a
+    // pinned stage, in flight until ev completes, then kept until release
+    // (moved to pageable memory when the pool already holds too much pinned)
     bool harvesting = false;
     void *stage = nullptr;
     size_t stage_bytes = 0;
+    std::vector<char> kept;
     hipEvent_t ev = nullptr;
-    // results, creation order
-    std::vector<double> x, v, f, e, vir, rho;
-    std::vector<int> tag, type, mask, image;
+    const void *results() const { return stage ? stage : (const void *)kept.data(); }
     long long step_end = 0;
     int nbuild = 0;
     long long pass_admit = 0;             // index of the job's first pass (its setup pass)
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
+
+// typed views of a finished job's stage (slot k holds the particle with
+// creation index origin[k], k < n)
+struct HarvestView {
+    const float4 *x, *v, *f;
+    const int *tag, *mask, *origin, *image;
+    const float *rho, *vir;   // vir: six planes of npad
+    const char *desc;
+    explicit HarvestView(const Job &j)
+    {
+        const size_t np = j.npad;
+        x = (const float4 *)j.results();
+        v = x + np;
+        f = x + 2 * np;
+        tag = (const int *)(x + 3 * np);
+        mask = tag + np;
+        origin = tag + 2 * np;
+        image = tag + 3 * np;
+        rho = (const float *)(tag + 4 * np);
+        vir = rho + np;
+        desc = (const char *)j.results() + np * HARVEST_SLOT_BYTES;
+    }
+};
+static_assert(HARVEST_SLOT_BYTES == 92, "stage layout");
+// the packed results fit the finished system's neighbor-row range, whose
+// width is at least NBR_ALIGN entries for at least 256 slots
+static_assert(sizeof(SysDev) <= 256 * (4 * NBR_ALIGN - HARVEST_SLOT_BYTES), "harvest block fits the rows");
 
 // bytes of device memory per particle slot (state and scratch), per cell
 // slot (rank, coordinates, system; worklist entry, linear range; count and
@@ -153,6 +181,7 @@ constexpr long long RETUNE_PASSES = 256;
 constexpr size_t ADMIT_LOOKAHEAD = 64;    // queued jobs considered per admission round
 constexpr int MAX_BATCH = 16;             // passes captured in one batch graph
 constexpr int MAX_BYPASS = 64;            // admissions past one waiting head job
+constexpr size_t MAX_HELD_PINNED = (size_t)1 << 30;  // pinned bytes of finished, unreleased jobs
 
 }  // namespace
 
@@ -211,6 +240,7 @@ struct mdpd_pool {
     int poll_cur = 0;
     bool need_regrow = false;
     PinnedCache pinned;
+    size_t held_pinned = 0;               // stage bytes kept by finished jobs
     std::vector<std::pair<hipEvent_t, std::pair<void *, size_t>>> uploads;
     std::vector<hipEvent_t> ev_free;
 
@@ -891,26 +921,15 @@ struct mdpd_pool {
         const bool uni = j.ntypes == 1;
         // energy / virial of the last pass; the launch whose split matches it runs
         for (int sp : {1, 2, 4}) hip_check(launch_force_range(P, style, MODE_HARVEST, uni, sp, t0, nt, stream), "harvest");
-        hip_check(launch_harvest(P, j.slot, (int)j.part_base, j.npad, stream), "harvest");
-        const size_t np = j.npad;
-        const size_t bytes = np * (3 * sizeof(float4) + 5 * sizeof(int) + 6 * sizeof(float)) + sizeof(SysDev);
+        // the results are packed into the system's own neighbor-row range
+        // (no pass reads it once the system has stopped) and copied as one
+        // block; they stay in the pinned stage until the job is released
+        const size_t bytes = harvest_bytes(j.npad);
+        int *dst = nlist.p + (size_t)t0 * maxnbr * WAVE;
+        const size_t room = (size_t)j.npad * maxnbr * sizeof(int);
+        hip_check(launch_harvest(P, j.slot, (int)j.part_base, j.npad, dst, room, stream), "harvest");
         j.stage = pinned.get(bytes, j.stage_bytes);
-        char *o = (char *)j.stage;
-        auto dn = [&](const void *src, size_t b) {
-            hip_check(hipMemcpyAsync(o, src, b, hipMemcpyDeviceToHost, stream), "harvest copy");
-            o += b;
-        };
-        const long long pb = j.part_base;
-        dn(pos[0].p + pb, np * sizeof(float4));
-        dn(vel[0].p + pb, np * sizeof(float4));
-        dn(frc.p + pb, np * sizeof(float4));
-        dn(tag[0].p + pb, np * sizeof(int));
-        dn(mask[0].p + pb, np * sizeof(int));
-        dn(origin[0].p + pb, np * sizeof(int));
-        dn(image[0].p + pb, np * sizeof(int));
-        dn(rho.p + pb, np * sizeof(float));
-        for (int c = 0; c < 6; ++c) dn(vir.p + (size_t)c * cap + pb, np * sizeof(float));
-        dn(dsys.p + j.slot, sizeof(SysDev));
+        hip_check(hipMemcpyAsync(j.stage, dst, bytes, hipMemcpyDeviceToHost, stream), "harvest copy");
         j.ev = take_event();
         hip_check(hipEventRecord(j.ev, stream), "hipEventRecord");
         j.harvesting = true;
@@ -928,44 +947,21 @@ struct mdpd_pool {
             const hipError_t q = block ? hipEventSynchronize(j.ev) : hipEventQuery(j.ev);
             if (q == hipErrorNotReady) continue;
             hip_check(q, "harvest event");
-            const size_t np = j.npad;
-            const char *o = (const char *)j.stage;
-            const float4 *hp = (const float4 *)o; o += np * sizeof(float4);
-            const float4 *hv = (const float4 *)o; o += np * sizeof(float4);
-            const float4 *hf = (const float4 *)o; o += np * sizeof(float4);
-            const int *ht = (const int *)o; o += np * sizeof(int);
-            const int *hm = (const int *)o; o += np * sizeof(int);
-            const int *ho = (const int *)o; o += np * sizeof(int);
-            const int *hi = (const int *)o; o += np * sizeof(int);
-            const float *hr = (const float *)o; o += np * sizeof(float);
-            const float *hw = (const float *)o; o += 6 * np * sizeof(float);
+            const HarvestView h(j);
+            for (int k = 0; k < j.n; ++k)
+                if (h.origin[k] < 0 || h.origin[k] >= j.n)
+                    fail(MDPD_E_HIP, "harvest of job %d: bad particle origin %d", j.id, h.origin[k]);
             SysDev d;
-            memcpy(&d, o, sizeof(SysDev));
-            const int n = j.n;
-            j.x.assign(3 * n, 0); j.v.assign(3 * n, 0); j.f.assign(3 * n, 0);
-            j.e.assign(n, 0); j.rho.assign(n, 0); j.vir.assign(6 * n, 0);
-            j.tag.assign(n, 0); j.type.assign(n, 0); j.mask.assign(n, 0); j.image.assign(3 * n, 0);
-            for (int k = 0; k < n; ++k) {
-                const int i = ho[k];
-                if (i < 0 || i >= n) fail(MDPD_E_HIP, "harvest of job %d: bad particle origin %d", j.id, i);
-                j.x[3 * i] = hp[k].x; j.x[3 * i + 1] = hp[k].y; j.x[3 * i + 2] = hp[k].z;
-                j.v[3 * i] = hv[k].x; j.v[3 * i + 1] = hv[k].y; j.v[3 * i + 2] = hv[k].z;
-                j.f[3 * i] = hf[k].x; j.f[3 * i + 1] = hf[k].y; j.f[3 * i + 2] = hf[k].z;
-                j.e[i] = hf[k].w;
-                j.rho[i] = hr[k];
-                for (int c = 0; c < 6; ++c) j.vir[6 * i + c] = hw[(size_t)c * np + k];
-                j.tag[i] = ht[k];
-                int ty;
-                memcpy(&ty, &hp[k].w, 4);
-                j.type[i] = ty + 1;
-                j.mask[i] = hm[k];
-                j.image[3 * i] = (hi[k] & 1023) - 512;
-                j.image[3 * i + 1] = ((hi[k] >> 10) & 1023) - 512;
-                j.image[3 * i + 2] = ((hi[k] >> 20) & 1023) - 512;
-            }
+            memcpy(&d, h.desc, sizeof(SysDev));
             j.step_end = d.step;
-            pinned.put(j.stage, j.stage_bytes);
-            j.stage = nullptr;
+            if (held_pinned + j.stage_bytes > MAX_HELD_PINNED) {
+                const char *b = (const char *)j.stage;
+                j.kept.assign(b, b + harvest_bytes(j.npad));
+                pinned.put(j.stage, j.stage_bytes);
+                j.stage = nullptr;
+            } else {
+                held_pinned += j.stage_bytes;
+            }
             ev_free.push_back(j.ev);
             j.ev = nullptr;
             // free the slot and ranges; tiles unmapped before any new occupant
@@ -1161,26 +1157,51 @@ int mdpd_pool_gather(mdpd_pool *pool, int job, const char *name, int type, int c
         Job &j = pool->job(job);
         if (j.state != JOB_DONE) fail(MDPD_E_STATE, "job %d has not finished", job);
         const std::string nm(name);
+        const HarvestView h(j);
+        const int n = j.n;
+        // checks the caller's type and width, then clears the output
         auto want = [&](int t, int c) {
             if (type != t || count != c) fail(MDPD_E_ARG, "gather of %s needs type %d count %d", name, t, c);
+            memset(data, 0, (size_t)n * c * (t == 1 ? sizeof(double) : sizeof(int)));
         };
-        const size_t n = j.n;
         if (nm == "x" || nm == "v" || nm == "f") {
             want(1, 3);
-            const std::vector<double> &s = nm == "x" ? j.x : (nm == "v" ? j.v : j.f);
-            memcpy(data, s.data(), sizeof(double) * 3 * n);
+            const float4 *src = nm == "x" ? h.x : (nm == "v" ? h.v : h.f);
+            double *o = (double *)data;
+            for (int k = 0; k < n; ++k) {
+                double *r = o + 3 * (size_t)h.origin[k];
+                r[0] = src[k].x; r[1] = src[k].y; r[2] = src[k].z;
+            }
         } else if (nm == "e_pair" || nm == "rho") {
             want(1, 1);
-            memcpy(data, (nm == "rho" ? j.rho : j.e).data(), sizeof(double) * n);
+            double *o = (double *)data;
+            if (nm == "rho")
+                for (int k = 0; k < n; ++k) o[h.origin[k]] = h.rho[k];
+            else
+                for (int k = 0; k < n; ++k) o[h.origin[k]] = h.f[k].w;
         } else if (nm == "virial") {
             want(1, 6);
-            memcpy(data, j.vir.data(), sizeof(double) * 6 * n);
+            double *o = (double *)data;
+            for (int k = 0; k < n; ++k)
+                for (int c = 0; c < 6; ++c) o[6 * (size_t)h.origin[k] + c] = h.vir[(size_t)c * j.npad + k];
         } else if (nm == "tag" || nm == "type" || nm == "mask") {
             want(0, 1);
-            memcpy(data, (nm == "tag" ? j.tag : (nm == "type" ? j.type : j.mask)).data(), sizeof(int) * n);
+            int *o = (int *)data;
+            for (int k = 0; k < n; ++k) {
+                int val;
+                if (nm == "tag") val = h.tag[k];
+                else if (nm == "mask") val = h.mask[k];
+                else { memcpy(&val, &h.x[k].w, 4); val += 1; }
+                o[h.origin[k]] = val;
+            }
         } else if (nm == "image") {
             want(0, 3);
-            memcpy(data, j.image.data(), sizeof(int) * 3 * n);
+            int *o = (int *)data;
+            for (int k = 0; k < n; ++k) {
+                const int im = h.image[k];
+                int *r = o + 3 * (size_t)h.origin[k];
+                r[0] = (im & 1023) - 512; r[1] = ((im >> 10) & 1023) - 512; r[2] = ((im >> 20) & 1023) - 512;
+            }
         } else {
             fail(MDPD_E_ARG, "unknown per-atom quantity '%s'", name);
         }
@@ -1193,7 +1214,21 @@ int mdpd_pool_thermo(mdpd_pool *pool, int job, double out[4])
         if (!out) fail(MDPD_E_ARG, "null output");
         Job &j = pool->job(job);
         if (j.state != JOB_DONE) fail(MDPD_E_STATE, "job %d has not finished", job);
-        thermo_of(j.n, j.mass, j.type.data(), j.v.data(), j.vir.data(), j.e.data(), j.volume, out);
+        // the per-atom arrays in creation order, as a handle sums them
+        const HarvestView h(j);
+        const int n = j.n;
+        std::vector<int> ty(n);
+        std::vector<double> v(3 * (size_t)n), vir(6 * (size_t)n), e(n);
+        for (int k = 0; k < n; ++k) {
+            const size_t i = h.origin[k];
+            int t;
+            memcpy(&t, &h.x[k].w, 4);
+            ty[i] = t + 1;
+            v[3 * i] = h.v[k].x; v[3 * i + 1] = h.v[k].y; v[3 * i + 2] = h.v[k].z;
+            for (int c = 0; c < 6; ++c) vir[6 * i + c] = h.vir[(size_t)c * j.npad + k];
+            e[i] = h.f[k].w;
+        }
+        thermo_of(n, j.mass, ty.data(), v.data(), vir.data(), e.data(), j.volume, out);
     });
 }
 
@@ -1212,6 +1247,11 @@ int mdpd_pool_release(mdpd_pool *pool, int job)
     POOL_GUARD(pool, {
         Job &j = pool->job(job);
         if (j.state != JOB_DONE) fail(MDPD_E_STATE, "job %d has not finished", job);
+        if (j.stage) {
+            pool->held_pinned -= j.stage_bytes;
+            pool->pinned.put(j.stage, j.stage_bytes);
+        }
+        j.stage = nullptr;
         pool->jobs.erase(job);
     });
 }
