@@ -744,6 +744,9 @@ __global__ void __launch_bounds__(256) k_commit_step(PoolDev P, int advance, int
 // its descriptor packed into one block at dst (x, v, f | tag, mask, origin,
 // image | rho, virial | SysDev, each array npad long), so the host copies one
 // range.  Runs after the HARVEST force pass, which reads the other buffer.
+// The system's tiles and cell slots are unmapped here (no kernel after it
+// finds the system through them); the host frees the ranges once the copy
+// has landed.
 // --------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_harvest(PoolDev P, int s, int pb, int npad, char *dst)
 {
@@ -754,6 +757,8 @@ __global__ void __launch_bounds__(256) k_harvest(PoolDev P, int s, int pb, int n
         int *o = (int *)(dst + (size_t)npad * HARVEST_SLOT_BYTES);
         for (int w = threadIdx.x; w < (int)(sizeof(SysDev) / 4); w += blockDim.x) o[w] = src[w];
     }
+    for (int c = k, nc = d->ncell + 1; c < nc; c += gridDim.x * blockDim.x) P.cell_sys[d->cell_base + c] = -1;
+    if (k < npad / WAVE) P.wave_sys[pb / WAVE + k] = -1;
     if (k >= npad) return;
     const int i = pb + k, cur = d->sel, cura = d->sela;
     const size_t n = npad;

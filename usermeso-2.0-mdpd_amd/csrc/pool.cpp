@@ -964,10 +964,8 @@ struct mdpd_pool {
             }
             ev_free.push_back(j.ev);
             j.ev = nullptr;
-            // free the slot and ranges; tiles unmapped before any new occupant
-            hip_check(hipMemsetAsync(wave_sys.p + j.part_base / WAVE, 0xFF, sizeof(int) * (j.npad / WAVE), stream),
-                      "memset");
-            hip_check(hipMemsetAsync(cell_sys.p + j.cell_base, 0xFF, sizeof(int) * (j.ncell + 1), stream), "memset");
+            // free the slot and ranges (the harvest kernel has unmapped its
+            // tiles and cell slots, ahead of any new occupant's upload)
             parts.give(j.part_base / 256, j.npad / 256);
             cells.give(j.cell_base, j.ncell + 1);
             slot_job[j.slot] = 0;
