@@ -201,7 +201,9 @@ int mdpd_tea_gaussian_noalloc(int device, int n, const unsigned int *a, const un
  * that does not fit yet lets later jobs that do fit pass it, at most 64
  * times before it goes first),
  * keeps its state resident while it runs, and leaves when done; its results
- * stay readable until mdpd_pool_release.  One host thread drives a pool. */
+ * stay readable until mdpd_pool_release (92 bytes per particle slot of host
+ * memory, pinned while finished jobs hold less than 1 GiB of it).  One host
+ * thread drives a pool. */
 typedef struct mdpd_pool mdpd_pool;
 
 enum { MDPD_JOB_QUEUED = 0, MDPD_JOB_RUNNING = 1, MDPD_JOB_DONE = 2 };
