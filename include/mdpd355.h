@@ -243,6 +243,9 @@ int mdpd_pool_release(mdpd_pool *pool, int job);
  * passes; the default), 1, 2 or 4;
  * "graph" 1/0: replay each pass as a captured hipGraph (default 1);
  * "poll": passes between status reads (default 8);
+ * "hold" MiB: pinned host memory finished, unreleased jobs may keep their
+ * results in (default 1024); past it a finished job's results move to
+ * pageable memory;
  * "profile" N: every N-th batch of passes is launched without the graph with
  * HIP events between its phases (0: off, the default; also resets the totals). */
 int mdpd_pool_tune(mdpd_pool *pool, const char *knob, int value);

@@ -75,6 +75,29 @@ def test_single_job_equals_plain_run(engine):
     assert p.stat("passes") >= 26
 
 
+def test_results_moved_to_pageable_memory_are_unchanged(engine):
+    # with no pinned memory allowed for finished jobs ("hold" 0) every result
+    # block is copied to pageable memory before it is read back
+    sizes = [(1296, 6.0, 11, 25), (700, 5.0, 12, 18)]
+    def run(hold):
+        p = engine.Pool(0.5 * GB, 16)
+        p.tune("split", 2)
+        p.tune("poll", 4)
+        p.tune("hold", hold)
+        ids = []
+        for n, L, seed, k in sizes:
+            x, v = uniform_box(n, L, seed)
+            ids.append(p.submit(mdpd_system(engine, x, v, L), k))
+        p.run()
+        out = [({k: p.gather(j, k) for k in ("x", "v", "f", "image", "e_pair", "virial", "rho", "tag", "type", "mask")},
+                p.thermo(j), p.info(j)["timestep"]) for j in ids]
+        p.close()
+        return out
+    pinned, pageable = run(1024), run(0)
+    for a, b in zip(pinned, pageable):
+        assert_same(a, b, "hold 0")
+
+
 def test_mixed_pool_equals_each_alone(engine):
     # mDPD droplets of several sizes, one with two types and a body force,
     # a DPD cube; different run lengths

@@ -181,7 +181,6 @@ constexpr long long RETUNE_PASSES = 256;
 constexpr size_t ADMIT_LOOKAHEAD = 64;    // queued jobs considered per admission round
 constexpr int MAX_BATCH = 16;             // passes captured in one batch graph
 constexpr int MAX_BYPASS = 64;            // admissions past one waiting head job
-constexpr size_t MAX_HELD_PINNED = (size_t)1 << 30;  // pinned bytes of finished, unreleased jobs
 
 }  // namespace
 
@@ -241,6 +240,7 @@ struct mdpd_pool {
     bool need_regrow = false;
     PinnedCache pinned;
     size_t held_pinned = 0;               // stage bytes kept by finished jobs
+    size_t max_held = (size_t)1 << 30;    // beyond this their results move to pageable memory
     std::vector<std::pair<hipEvent_t, std::pair<void *, size_t>>> uploads;
     std::vector<hipEvent_t> ev_free;
 
@@ -954,7 +954,7 @@ struct mdpd_pool {
             SysDev d;
             memcpy(&d, h.desc, sizeof(SysDev));
             j.step_end = d.step;
-            if (held_pinned + j.stage_bytes > MAX_HELD_PINNED) {
+            if (held_pinned + j.stage_bytes > max_held) {
                 const char *b = (const char *)j.stage;
                 j.kept.assign(b, b + harvest_bytes(j.npad));
                 pinned.put(j.stage, j.stage_bytes);
@@ -1268,6 +1268,9 @@ int mdpd_pool_tune(mdpd_pool *pool, const char *knob, int value)
             pool->graph_valid = false;
         } else if (k == "graph") {
             pool->use_graph = value != 0;
+        } else if (k == "hold") {
+            if (value < 0) fail(MDPD_E_ARG, "hold must be >= 0 MiB");
+            pool->max_held = (size_t)value << 20;
         } else if (k == "profile") {
             if (value < 0) fail(MDPD_E_ARG, "profile must be >= 0");
             pool->prof_every = value;
