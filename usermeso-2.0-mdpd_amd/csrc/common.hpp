@@ -269,8 +269,8 @@ __device__ inline uint32_t sig_dpd(int tag, uint32_t seed_t, float vx, float vy,
 //  * cos(pi x) = -sin(pi t), t = x - 1/2 in [-1/2, 1/2), with the Taylor
 //    series of sin to t^11 (truncation < 6e-8);
 //  * ln u for u = v1 / 2^32: for v1 >= 2^31 through log1p of the exact
-//    complement (Kahan's log1p identity, so u near 1 keeps its precision),
-//    otherwise log2 - 32 (|log2 u| >= 1, no cancellation).
+//    complement (u near 1 keeps its precision), otherwise the log2 of
+//    u itself (|log2 u| >= 1).
 // ----------------------------------------------------------------------------
 
 __device__ inline float sinpi_half(float t)
@@ -286,23 +286,24 @@ __device__ inline float sinpi_half(float t)
     return s * t;
 }
 
-// ln(w / 2^32) for 1 <= w < 2^32 without branches: with the leading-bit
-// exponent e (2^e <= w < 2^(e+1)) pick k in {e, e+1} so that a = w / 2^k lies
-// in [1/sqrt2, sqrt2); f = a - 1 is formed from the exact integer difference
-// w - 2^k, and ln u = (k - 32) ln 2 + log1p(f) with Kahan's identity
-// log1p(f) = ln(1+f) * f / ((1+f) - 1), so u near 1 keeps full precision.
+// ln(w / 2^32) for 1 <= w < 2^32 with one hardware log2 and no branches.
+// w < 2^31 (u < 1/2): u = w 2^-32 is a normal float (w rounded to 24 bits,
+// the power-of-two scale exact) and |log2 u| >= 1, so log2 u is accurate
+// relative to itself.  w >= 2^31: u = 1 + y with y = -(2^32 - w) 2^-32 from
+// the exact integer complement; w1 = fl(1 + y) in [1/2, 1], and
+// ln u = ln w1 + ln(1 + (u - w1) / w1) ~ ln w1 + (y - (w1 - 1)), where
+// w1 - 1 and y - (w1 - 1) are exact (Sterbenz) and the neglected factor
+// 1/w1 - 1 <= 1 - w1 scales a term of at most 2^-25, so u near 1 keeps its
+// relative precision (log1p of the complement, the correction that
+// Kahan's log1p identity applies as a quotient).
 __device__ inline float ln_u32(uint32_t w)
 {
-    const int e = 31 - (int)__builtin_clz(w);
-    const bool up = __builtin_amdgcn_ldexpf((float)w, -e) >= 1.41421356f;
-    const int k = up ? e + 1 : e;
-    const uint32_t pk = k >= 32 ? 0u : (1u << k);
-    const float f = __builtin_amdgcn_ldexpf((float)(int)(w - pk), -k);
-    const float w1 = 1.0f + f;
-    // w1 lies in [1/sqrt2, sqrt2]: the hardware log2 needs no range or
-    // denormal handling there
-    const float lp = __builtin_amdgcn_logf(w1) * 0.69314718055994530942f * f * __builtin_amdgcn_rcpf(w1 - 1.0f);
-    return (float)(k - 32) * 0.69314718055994530942f + (w1 == 1.0f ? f : lp);
+    const bool hi = w >= 0x80000000u;
+    const float y = (float)(0u - w) * -2.3283064365386963e-10f;   // -(2^32 - w) 2^-32
+    const float w1 = 1.0f + y;
+    const float arg = hi ? w1 : (float)w * 2.3283064365386963e-10f;
+    const float corr = hi ? y - (w1 - 1.0f) : 0.0f;
+    return fmaf(__builtin_amdgcn_logf(arg), 0.69314718055994530942f, corr);
 }
 
 __device__ inline float pair_gaussian(uint32_t si, uint32_t sj)
