@@ -268,9 +268,9 @@ __device__ inline uint32_t sig_dpd(int tag, uint32_t seed_t, float vx, float vy,
 // cos(pi x) and ln u are evaluated in FP32 from the mathematics:
 //  * cos(pi x) = -sin(pi t), t = x - 1/2 in [-1/2, 1/2), with the Taylor
 //    series of sin to t^11 (truncation < 6e-8);
-//  * ln u for u = v1 / 2^32: for v1 >= 2^31 through log1p of the exact
-//    complement (u near 1 keeps its precision), otherwise the log2 of
-//    u itself (|log2 u| >= 1).
+//  * ln u for u = v1 / 2^32: the log2 of u rounded to FP32, corrected to
+//    first order by u's rounding error from the exact integer complement
+//    2^32 - v1 (u near 1 keeps its precision).
 // ----------------------------------------------------------------------------
 
 __device__ inline float sinpi_half(float t)
@@ -286,24 +286,20 @@ __device__ inline float sinpi_half(float t)
     return s * t;
 }
 
-// ln(w / 2^32) for 1 <= w < 2^32 with one hardware log2 and no branches.
-// w < 2^31 (u < 1/2): u = w 2^-32 is a normal float (w rounded to 24 bits,
-// the power-of-two scale exact) and |log2 u| >= 1, so log2 u is accurate
-// relative to itself.  w >= 2^31: u = 1 + y with y = -(2^32 - w) 2^-32 from
-// the exact integer complement; w1 = fl(1 + y) in [1/2, 1], and
-// ln u = ln w1 + ln(1 + (u - w1) / w1) ~ ln w1 + (y - (w1 - 1)), where
-// w1 - 1 and y - (w1 - 1) are exact (Sterbenz) and the neglected factor
-// 1/w1 - 1 <= 1 - w1 scales a term of at most 2^-25, so u near 1 keeps its
-// relative precision (log1p of the complement, the correction that
-// Kahan's log1p identity applies as a quotient).
-__device__ inline float ln_u32(uint32_t w)
+// -2 ln u for u = w / 2^32, 1 <= w < 2^32, with one hardware log2 and no
+// branches.  u_f = fl(w) 2^-32 is a normal float (the power-of-two scale is
+// exact) and ln u = ln u_f + ln(u / u_f) ~ ln u_f + (u - u_f), where
+// u - u_f = (1 - u_f) - c 2^-32 with the exact integer complement
+// c = 2^32 - w.  For u_f >= 1/2, 1 - u_f is exact (Sterbenz) and the
+// neglected factor 1/u_f - 1 <= 1 - u_f scales a term of at most 2^-25, so
+// u near 1, where ln u ~ -(1 - u), keeps its relative precision; below 1/2,
+// |ln u| > 0.69 and the correction stays below 2^-24.
+__device__ inline float neg2_ln_u32(uint32_t w)
 {
-    const bool hi = w >= 0x80000000u;
-    const float y = (float)(0u - w) * -2.3283064365386963e-10f;   // -(2^32 - w) 2^-32
-    const float w1 = 1.0f + y;
-    const float arg = hi ? w1 : (float)w * 2.3283064365386963e-10f;
-    const float corr = hi ? y - (w1 - 1.0f) : 0.0f;
-    return fmaf(__builtin_amdgcn_logf(arg), 0.69314718055994530942f, corr);
+    const float uf = (float)w * 2.3283064365386963e-10f;                       // w 2^-32
+    // -2 (u - u_f) = 2 u_f - 2 + c 2^-31
+    const float e = fmaf((float)(0u - w), 4.6566128730773926e-10f, fmaf(uf, 2.0f, -2.0f));
+    return fmaf(__builtin_amdgcn_logf(uf), -1.3862943611198906f, e);           // -2 ln 2
 }
 
 __device__ inline float pair_gaussian(uint32_t si, uint32_t sj)
@@ -316,7 +312,7 @@ __device__ inline float pair_gaussian(uint32_t si, uint32_t sj)
     const float t = (float)ti * 4.656612873077392578125e-10f;
     const float s = sinpi_half(t);                 // cos(pi x) = -sin(pi t)
     const float f = (v0 & 0x80000000u) ? -s : s;   // times +1 / -1 by the top bit
-    const float r = __builtin_amdgcn_sqrtf(-2.0f * ln_u32(v1 > 1u ? v1 : 1u));
+    const float r = __builtin_amdgcn_sqrtf(neg2_ln_u32(v1 > 1u ? v1 : 1u));
     return fminf(fmaxf(r * f, -4.0f), 4.0f);
 }
 

@@ -72,11 +72,15 @@ __device__ inline void pair_delta(const float4 &pi, const float4 &q, uint32_t co
     }
 }
 
+// r^2 with the fusion spelled out, so the first pass's compaction test and
+// the force pass's cutoff test see the same bits
+__device__ inline float dist2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
+
 __device__ inline float pair_rsq(const float4 &pi, const float4 &q, uint32_t code, const SysDev *d)
 {
     float dx, dy, dz;
     pair_delta(pi, q, code, d, dx, dy, dz);
-    return dx * dx + dy * dy + dz * dz;
+    return dist2(dx, dy, dz);
 }
 
 // --------------------------------------------------------------------------
@@ -130,7 +134,7 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_first(PoolDev P, int ntiles)
             const float hm = h - r;
             sum += lucy ? (h + 3.f * r) * hm * hm * hm : hm * hm;
         }
-        if (rsq < rc2) {
+        if (rsq < rc2 && rsq >= 1e-20f) {
             if constexpr (decltype(room)::value) {
                 s_c[w][nc * WAVE + lane] = (int)e;
             } else {
@@ -297,15 +301,19 @@ struct Acc {
     __device__ Acc() { for (int k = 0; k < N; ++k) v[k] = 0.f; }
 };
 
-template <int STYLE, bool EV>
+// TEST = false: the entry is known to pass the cutoff test (a one-type
+// system's compact row holds exactly the entries with
+// 1e-20 <= r^2 < rc^2, r^2 computed from the same positions by dist2, and
+// the system's largest cutoff is its only one)
+template <int STYLE, bool EV, bool TEST = true>
 __device__ inline void pair_term(Acc<EV> &a, const float4 &pi, const float4 &vi, float rho_i, const float4 &q,
                                  const float4 &vj, float rho_j, uint32_t code, const PairC &c, float dtis,
                                  const SysDev *d)
 {
     float dx, dy, dz;
     pair_delta(pi, q, code, d, dx, dy, dz);
-    const float rsq = dx * dx + dy * dy + dz * dz;
-    const bool in = rsq < c.rc2 && rsq >= 1e-20f;
+    const float rsq = dist2(dx, dy, dz);
+    const bool in = !TEST || (rsq < c.rc2 && rsq >= 1e-20f);
     const float rinv = __builtin_amdgcn_rsqf(rsq);
     const float r = rsq * rinv;
     const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
@@ -445,7 +453,7 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
                 for (int u = 0; u < 4; ++u) {
                     PairC c = cu;
                     wall_scale(c, j[u]);
-                    pair_term<STYLE, EV>(a, pi, vi, rho_i, q[u], vv[u], rj[u], e[u] >> IMG_SHIFT, c, dtis, d);
+                    pair_term<STYLE, EV, !UNI>(a, pi, vi, rho_i, q[u], vv[u], rj[u], e[u] >> IMG_SHIFT, c, dtis, d);
                 }
             }
         }
@@ -462,8 +470,8 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
             PairC c1 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q1.w));
             wall_scale(c0, j0);
             wall_scale(c1, j1);
-            pair_term<STYLE, EV>(a, pi, vi, rho_i, q0, w0, r0, e0 >> IMG_SHIFT, c0, dtis, d);
-            pair_term<STYLE, EV>(a, pi, vi, rho_i, q1, w1, r1, e1 >> IMG_SHIFT, c1, dtis, d);
+            pair_term<STYLE, EV, !UNI>(a, pi, vi, rho_i, q0, w0, r0, e0 >> IMG_SHIFT, c0, dtis, d);
+            pair_term<STYLE, EV, !UNI>(a, pi, vi, rho_i, q1, w1, r1, e1 >> IMG_SHIFT, c1, dtis, d);
         }
         if (t < end) {
             const uint32_t e0 = (uint32_t)crow[(size_t)(t * SPLIT + w) * WAVE];
@@ -473,7 +481,7 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
             nbr(j0, q0, r0);
             PairC c0 = UNI ? cu : load_coef(coef, ti * nt + __float_as_int(q0.w));
             wall_scale(c0, j0);
-            pair_term<STYLE, EV>(a, pi, vi, rho_i, q0, vel[j0], r0, e0 >> IMG_SHIFT, c0, dtis, d);
+            pair_term<STYLE, EV, !UNI>(a, pi, vi, rho_i, q0, vel[j0], r0, e0 >> IMG_SHIFT, c0, dtis, d);
         }
     };
     // Lane balance (one wave per tile, one type, no wall scaling): a wave
@@ -518,7 +526,7 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
                 const int t = M + f - __shfl(first, p);
                 const bool act = f < tail;
                 // a lane past the last tail evaluates its own particle with
-                // itself (r = 0: no force) and stores nothing
+                // itself (r = 0: terms not finite) and stores nothing
                 const uint32_t e = act ? (uint32_t)ctile[(size_t)t * WAVE + p]
                                        : ((uint32_t)i | ((uint32_t)IMG_ZERO << IMG_SHIFT));
                 const uint32_t j = e & IDX_MASK;
@@ -526,7 +534,7 @@ __global__ void __launch_bounds__(WAVE * SPLIT) k_force(PoolDev P, int tile0, in
                 float rj;
                 nbr(j, q, rj);
                 Acc<EV> b;
-                pair_term<STYLE, EV>(b, s_pi[p], s_vi[p], s_ri[p], q, vel[j], rj, e >> IMG_SHIFT, cu, dtis, d);
+                pair_term<STYLE, EV, !UNI>(b, s_pi[p], s_vi[p], s_ri[p], q, vel[j], rj, e >> IMG_SHIFT, cu, dtis, d);
                 if (act)
                     for (int k = 0; k < NA; ++k) s_tail[k * TAIL_CAP + f] = b.v[k];
             }
