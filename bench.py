@@ -62,7 +62,7 @@ B_REBUILD = 52 + 52 + 16 + 4 * NLIST + 4                     # 347 B per particl
 # HBM bytes per particle moved by the pooled force pass, measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (medians over the passes of 64
 # droplets, counters scaled by a 1 GiB copy: x2.0 fetch, x1.0 write)
-T_FORCE_POOL = 175.1 + 32.1
+T_FORCE_POOL = 174.3 + 32.1
 GB = float(1 << 30)
 
 SLABS = {
