@@ -29,6 +29,7 @@ __host__ __device__ inline bool nlist_listed(const PoolDev &P) { return P.nsys >
 __global__ void __launch_bounds__(256) k_keys(PoolDev P)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && P.work_count) *P.work_count = 0;        // this rebuild's worklist starts empty
     if (i >= P.nact) return;
     const int s = P.wave_sys[i >> 6];
     if (s < 0) return;
@@ -539,10 +540,25 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8, 8)
     if (cs < ncells) nlist_cell<false>(P, make_int4(cs, 0, 0, 0), relist, s_row);
 }
 
-// the worklist walk
+// Commit after a rebuild: the sorted data in the other buffers becomes current.
+__device__ inline void commit_rebuild_sys(const PoolDev &P, int s)
+{
+    SysDev *d = P.sys + s;
+    if (!d->active || !d->stale) return;
+    d->sel ^= 1;
+    d->sela ^= 1;
+    d->stale = 0;
+    d->ago = 0;
+    d->builds += 1;
+}
+
+// the worklist walk; it also commits the rebuild of every re-sorted system
+// (the row build reads only their geometry, and the next kernel sees the
+// flipped buffers), which saves a launch per pass
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(NL_WAVES, NL_WAVES))) k_nlist_listed(PoolDev P)
 {
     __shared__ int s_row[NL_ITER * WAVE];
+    for (int s = blockIdx.x * WAVE + threadIdx.x; s < P.nsys; s += gridDim.x * WAVE) commit_rebuild_sys(P, s);
     // gridDim.x is a multiple of NXCD; workgroup b runs on XCD b % NXCD
     const int n = *P.work_count;
     const int x = blockIdx.x % NXCD, per = gridDim.x / NXCD;
@@ -562,20 +578,12 @@ __global__ void k_clear_overflow(PoolDev P)
 }
 
 // --------------------------------------------------------------------------
-// Commit after a rebuild: the sorted data in the other buffers becomes current.
+// Commit after a rebuild with one wave per cell slot (the plain path)
 // --------------------------------------------------------------------------
 __global__ void k_commit_rebuild(PoolDev P)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0) *P.work_count = 0;                         // the next rebuild's worklist starts empty
-    if (s >= P.nsys) return;
-    SysDev *d = P.sys + s;
-    if (!d->active || !d->stale) return;
-    d->sel ^= 1;
-    d->sela ^= 1;
-    d->stale = 0;
-    d->ago = 0;
-    d->builds += 1;
+    if (s < P.nsys) commit_rebuild_sys(P, s);
 }
 
 // --------------------------------------------------------------------------
@@ -647,6 +655,8 @@ hipError_t launch_relist(const PoolDev &P, hipStream_t st)
 
 hipError_t launch_commit_rebuild(const PoolDev &P, hipStream_t st)
 {
+    // after a worklist walk the commit has been made by the walk itself
+    if (nlist_listed(P) && P.ncact > 0 && P.nact > 0) return hipSuccess;
     hipLaunchKernelGGL(k_commit_rebuild, dim3(nblk(P.nsys, 64)), dim3(64), 0, st, P);
     return hipGetLastError();
 }
