@@ -238,6 +238,34 @@ def test_many_resident_jobs_beyond_one_launch_block(engine):
         assert_same(got[k], plain_result(mk(k), nsteps[k]), f"job {k}")
 
 
+def test_full_batch_of_256_droplets(engine):
+    # the batched workload at its full size: 256 droplets with N ~
+    # LogNormal(ln 8000, 0.5) clipped to [2000, 40000] (~2.4M particles) in
+    # one pool for 20 steps.  Size-independent checks: every job reaches
+    # step 20 with finite state and all of its atoms, its net momentum stays
+    # at FP32 rounding level, and the smallest, a middle and the largest job
+    # equal the same systems run alone, bit for bit
+    rng = np.random.default_rng(20241202)
+    sizes = np.clip(np.exp(rng.normal(np.log(8000.0), 0.5, 256)), 2000, 40000).astype(int)
+    assert 2.0e6 < sizes.sum() < 2.8e6
+    mk = lambda k: droplet(engine, int(sizes[k]), 7000 + k)
+    got, p = pool_results(engine, [mk(k) for k in range(256)], [20] * 256, budget=16 * GB, max_systems=256)
+    worst = 0.0
+    for k, (r, th, ts) in enumerate(got):
+        assert ts == 20, k
+        assert r["x"].shape == (sizes[k], 3), k
+        for key in ("x", "v", "f", "e_pair", "virial"):
+            assert np.isfinite(r[key]).all(), (k, key)
+        # the droplets start at zero net momentum (to FP32 rounding)
+        worst = max(worst, np.abs(r["v"].sum(axis=0)).max() / np.sqrt(sizes[k]))
+    print("largest |net momentum| / sqrt(N) after 20 steps", worst)
+    assert worst < 2e-5
+    order = np.argsort(sizes)
+    for k in (order[0], order[128], order[-1]):
+        assert_same(got[k], plain_result(mk(k), 20), f"job {k} (N = {sizes[k]})")
+    p.close()
+
+
 def test_job_finished_before_a_regrow_keeps_its_last_pass(engine):
     # job a finishes on the device inside a batch of passes whose status word
     # the host has not read yet (it reads them one batch behind); job b,
