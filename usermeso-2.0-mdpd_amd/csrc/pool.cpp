@@ -937,7 +937,8 @@ struct mdpd_pool {
         ++nharvest;
     }
 
-    // finished copies -> results in creation order; ranges and slot freed
+    // landed copies: the job is done (its packed results stay in the stage),
+    // its ranges and slot are freed
     int finish_harvests(bool block)
     {
         int done = 0;
