@@ -253,7 +253,8 @@ int mdpd_pool_tune(mdpd_pool *pool, const char *knob, int value);
  * (particle slots), maxnbr, bytes (device bytes held), regrows, running,
  * queued, used_slots, tune_ms_1 / tune_ms_2 / tune_ms_4 (ms of the first and
  * force passes per pass, the split-dependent part the autotuner compares),
- * prof_passes and prof_{rebuild,first,force,commit}_ms (sampled totals). */
+ * prof_passes and prof_{rebuild,first,force,commit}_ms (sampled totals),
+ * held_pinned (pinned host bytes holding finished, unreleased results). */
 int mdpd_pool_stat(mdpd_pool *pool, const char *name, double *value);
 /* Monotonic clock of the job times, in seconds. */
 double mdpd_clock(void);

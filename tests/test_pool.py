@@ -91,9 +91,15 @@ def test_results_moved_to_pageable_memory_are_unchanged(engine):
         p.run()
         out = [({k: p.gather(j, k) for k in ("x", "v", "f", "image", "e_pair", "virial", "rho", "tag", "type", "mask")},
                 p.thermo(j), p.info(j)["timestep"]) for j in ids]
+        # finished jobs hold pinned memory only below the limit, none after release
+        assert (p.stat("held_pinned") > 0) == (hold > 0)
+        for j in ids:
+            p.release(j)
+        assert p.stat("held_pinned") == 0
         p.close()
         return out
     pinned, pageable = run(1024), run(0)
+    assert all(np.isfinite(a[0]["x"]).all() for a in pinned)
     for a, b in zip(pinned, pageable):
         assert_same(a, b, "hold 0")
 

@@ -1307,6 +1307,7 @@ int mdpd_pool_stat(mdpd_pool *pool, const char *name, double *value)
         else if (k == "prof_passes") *value = (double)pool->prof_passes;
         else if (k == "prof_particles") *value = pool->prof_particles;
         else if (k == "particle_builds") *value = pool->particle_builds;
+        else if (k == "held_pinned") *value = (double)pool->held_pinned;
         else if (k == "prof_rebuild_ms") *value = pool->prof_ms[mdpd_pool::PP_REBUILD];
         else if (k == "prof_first_ms") *value = pool->prof_ms[mdpd_pool::PP_FIRST];
         else if (k == "prof_force_ms") *value = pool->prof_ms[mdpd_pool::PP_FORCE];
