@@ -87,6 +87,25 @@ def test_two_particles_match_oracle(engine, orc):
     np.testing.assert_allclose(s.gather("f").sum(axis=0), 0, atol=1e-5)
 
 
+def test_coincident_particles_are_left_out(engine, orc):
+    # particles at exactly the same position: each lists the other (a
+    # neighbor is left out of its own row by identity, not by distance), and
+    # the density and force passes leave the pair out (r^2 < 1e-20,
+    # fix_aboundary_rho_meso.cu:155, pair_mdpd_meso.cu:140); a short run
+    # separates them with finite forces
+    L, n = 4.0, 400
+    x, v = uniform_box(n, L, 57)
+    for a, b in ((0, 1), (10, 11), (20, 21), (30, 31), (30, 32)):
+        x[b] = x[a]
+    s = mdpd_system(engine, x, v, L)
+    s.setup()
+    f, _ = _check_against_oracle(engine, orc, s, L, n, "coincident")
+    assert_vec_close(s.gather("f"), f, what="coincident pairs: force")
+    s.run(5)
+    for k in ("x", "v", "f", "rho"):
+        assert np.isfinite(s.gather(k)).all(), k
+
+
 def test_particles_on_the_box_faces(engine, orc):
     # positions exactly on the faces, the float just below L, tiny negatives
     # that round onto L after the wrap, and whole periods outside: wrap, keys,
