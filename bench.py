@@ -617,6 +617,9 @@ def run_headline(a, legs):
         for name in ("slab-literal", "slab"):
             log(f"slab: {name}")
             config["slab"].append(slab_record(name, a.slab_steps))
+    if "wall" in legs:
+        log("wall")
+        config["wall"] = wall_record(a.job_steps)
     if "cpu" in legs:
         med = int(np.argsort(sizes)[len(sizes) // 2])
         box, x, v, _ = specs[med]
@@ -711,6 +714,119 @@ def run_pore(a):
     print(json.dumps(base_line(a, value, dt / a.steps * 1e3, config, roof, None)))
 
 
+# ---------------------------------------------------------------------------
+# wall: the arbitrary-boundary branch of fix aboundary/rho/meso (wall
+# fractions, wall-scaled pair forces, bounce-back) on the plain path
+# ---------------------------------------------------------------------------
+WALL_BOX = (40.0, 12.0, 40.0)
+WALL_DECK = """
+    neighbor {skin} bin
+    neigh_modify every 1 delay 0 check yes
+    mass * 1.0
+    group fluid type 1
+    group wall type 2
+    pair_style mdpd/meso 1.0 {rc} 2654868 fluid wall
+    pair_coeff 1 1 -40 25 18.0 {rc} {rd}
+    pair_coeff 1 2 -20 25 18.0 {rc} {rd}
+    pair_coeff 2 2 -40 25 18.0 {rc} {rd}
+    fix BC all aboundary/rho/meso cut_rho {rd} phi_c 0.5 cut_phi 1.0 rho_wall 8.0 wall wall mobile fluid
+    fix int fluid nve/meso
+    timestep 0.01
+    run_style mvv/meso
+"""
+
+
+def make_channel(seed=20240605):
+    """Slit channel: frozen wall particles (type 2) on a simple cubic lattice of
+    spacing 1/2 (density 8) in y in [0, 1) and [11, 12), fluid (type 1) at
+    density 6.05 uniform in y in [1, 11) with Maxwell velocities."""
+    lx, ly, lz = WALL_BOX
+    rng = np.random.default_rng(seed)
+    g = lambda lo, hi: np.arange(lo, hi, 0.5) + 0.25
+    walls = np.stack(np.meshgrid(g(0.0, lx), np.r_[g(0.0, 1.0), g(ly - 1.0, ly)], g(0.0, lz), indexing="ij"),
+                     -1).reshape(-1, 3)
+    nf = int(round(RHO * lx * (ly - 2.0) * lz))
+    fluid = np.c_[rng.uniform(0, lx, nf), rng.uniform(1.0, ly - 1.0, nf), rng.uniform(0, lz, nf)]
+    v = np.zeros((nf + len(walls), 3))
+    v[:nf] = _mb(rng, nf)
+    types = np.r_[np.ones(nf, np.int32), np.full(len(walls), 2, np.int32)]
+    return _f32(np.r_[fluid, walls]), _f32(v), types, nf
+
+
+def wall_record(steps, profile_steps=200, split=None):
+    import mdpd355
+    x, v, types, nf = make_channel()
+    n = len(x)
+
+    def handle(sp):
+        s = mdpd355.System(0)
+        s.command("atom_style mdpd/atomic/meso 8.0")
+        s.create_box(2, (0, 0, 0), WALL_BOX)
+        s.create_atoms(types, x, v)
+        s.commands(WALL_DECK.format(skin=SKIN, rc=RC, rd=RD))
+        if sp:
+            s.tune("split", sp)
+        if not PLAIN_GRAPH[0]:
+            s.tune("graph", 0)
+        return s
+
+    def timed_run(sp, k):
+        s = handle(sp)
+        s.setup()
+        t0 = time.perf_counter()
+        s.run(k)
+        dt = time.perf_counter() - t0
+        out = (dt, s.nbuild, s.thermo()["temp"])
+        s.close()
+        return out
+
+    if split is None:
+        times = {sp: timed_run(sp, 100)[0] for sp in (1, 2, 4)}
+        split = min(times, key=times.get)
+    timed_run(split, 20)                                            # warm-up
+    dt, nb, temp = timed_run(split, steps)
+    s = handle(split)
+    s.setup()
+    origin = s.debug_sorted()[0]
+    counts = np.asarray(s.debug_neighbors()[0])[:n]
+    nrow = float(np.mean(counts[np.asarray(origin)[:n] < nf]))        # rows of the fluid particles
+    s.profile(True)
+    s.run(profile_steps)
+    prof = {ph: s.profile_get(ph) for ph in ("force", "density", "rebuild", "initial", "wall")}
+    s.close()
+    value = n * steps / dt
+    w_ms, w_n = prof["wall"]
+    t_wall = w_ms / max(w_n, 1) * 1e-3
+    # the wall pass per fluid particle: its full row (4 B per entry), position,
+    # mask and build position read, wall fraction written
+    b_wall = 4.0 * nrow + 16 + 4 + 16 + 4
+    return {
+        "workload": f"plain mDPD slit channel with the arbitrary-boundary branch: box "
+                    f"{WALL_BOX[0]:g}x{WALL_BOX[1]:g}x{WALL_BOX[2]:g}, frozen lattice walls (density 8) in y in "
+                    f"[0,1) and [11,12), fluid at density 6.05 between, N={n} ({n - nf} wall), A11=A22=-40 "
+                    f"A12=-20 B=25 gamma=18 rc=1 rd=0.75, phi_c 0.5, cut_phi 1, rho_wall 8, dt=0.01, skin 0.3, "
+                    f"{steps} steps",
+        "particles": n, "wall_particles": n - nf, "steps": steps, "value": value, "timed_region_s": dt,
+        "split": split, "builds": nb,
+        "phase_ms_per_step": {k: ms / max(profile_steps, 1) for k, (ms, _) in prof.items()},
+        "wall_pass_us": t_wall * 1e6, "mean_row_entries_fluid": nrow, "wall_pass_bytes_per_fluid_particle": b_wall,
+        "wall_pass_GBps": b_wall * nf / t_wall / 1e9 if t_wall > 0 else None,
+        "final_temp": temp,
+    }
+
+
+def run_wall(a):
+    r = wall_record(a.job_steps)
+    log(f"wall: {r['value']:.3e} particle-steps/s, wall pass {r['wall_pass_us']:.1f} us")
+    achieved = r["wall_pass_GBps"]
+    roof = {"bound": "hbm", "kernel": "k_aboundary (wall fractions and bounce-back)", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "traffic": None, "bytes_per_particle": r["wall_pass_bytes_per_fluid_particle"],
+            "particles_per_launch": r["particles"] - r["wall_particles"], "avg_launch_us": r["wall_pass_us"]}
+    config = dict(r, mode="plain")
+    print(json.dumps(base_line(a, r["value"], r["timed_region_s"] * 1e3 / a.steps, config, roof, None)))
+
+
 def run_dpd(a):
     """Config 1: the classic DPD fluid (L = 22, N = 32,000, a = 25, gamma = 4.5,
     dt = 0.04) on the GPU (dpd/meso, sigma = 3) beside the CPU LAMMPS
@@ -792,9 +908,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5, help="untimed passes of a warm-up pool")
     ap.add_argument("--job-steps", type=int, default=2000, help="steps of every batched / served job")
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "batched", "served", "pore", "dpd"] + sorted(SLABS))
-    ap.add_argument("--legs", default="batched,slab,cpu,served",
-                    help="legs of the headline workload (batched, slab, cpu, served)")
+                    choices=["headline", "batched", "served", "pore", "dpd", "wall"] + sorted(SLABS))
+    ap.add_argument("--legs", default="batched,slab,wall,cpu,served",
+                    help="legs of the headline workload (batched, slab, wall, cpu, served)")
     ap.add_argument("--jobs", type=int, default=256)
     ap.add_argument("--seed", type=int, default=20241202)
     ap.add_argument("--budget-gb", type=float, default=16.0, help="HBM budget of the batched pool")
@@ -841,6 +957,8 @@ def main():
         run_headline(a, {"served"})
     elif a.workload == "pore":
         run_pore(a)
+    elif a.workload == "wall":
+        run_wall(a)
     elif a.workload == "dpd":
         run_dpd(a)
     else:

@@ -145,7 +145,8 @@ int mdpd_tune(mdpd_system *sys, const char *knob, int value);
  * HIP events recorded around every launch on the handle's stream, folded into
  * per-phase totals at the end of each setup / run.  Phases: "rebuild"
  * (wrap, keys, sort, gather, neighbor rows), "density", "force" (pair force
- * with the fused integration), "initial" (first half-kick of a run).  The
+ * with the fused integration), "initial" (first half-kick of a run), "wall"
+ * (wall fractions and bounce-back of fix aboundary/rho/meso's wall branch).  The
  * reference's counterpart is MesoTimer::device_stamp and the -profile switch
  * (src/USER-MESO/timer_meso.cu:14-19, src/lammps.cpp:189-191). */
 int mdpd_profile(mdpd_system *sys, int enable);          /* also resets the totals */

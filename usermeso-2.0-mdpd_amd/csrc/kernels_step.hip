@@ -630,23 +630,50 @@ __global__ void __launch_bounds__(WAVE) k_aboundary(PoolDev P, int ntiles, int p
         if (mask[i] & d->fix_mobile_bit) {
             const int maxnbr = P.maxnbr;
             const int cnt = min(P.ncount[i], maxnbr);
-            const int *row = P.nlist + nbr_at(i, 0, maxnbr);
+            const int4 *row4 = (const int4 *)(P.nlist + nbr_at(i, 0, maxnbr));
             const float c = d->cut_phi, c2 = c * c;
+            const int wbit = d->fix_wall_bit;
             float phi = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
-            for (int q = 0; q < cnt; ++q) {
-                const uint32_t e = (uint32_t)row[(size_t)(q & ~(NBR_CHUNK - 1)) * WAVE + (q & (NBR_CHUNK - 1))];
-                const uint32_t j = e & IDX_MASK;
-                if (!(mask[j] & d->fix_wall_bit)) continue;
-                float dx, dy, dz;
-                pair_delta(p, pos[j], e >> IMG_SHIFT, d, dx, dy, dz);
-                const float rsq = dx * dx + dy * dy + dz * dz;
-                if (!(rsq < c2)) continue;
-                const float rinv = __builtin_amdgcn_rsqf(rsq);
-                const float r = fmaxf(rsq * rinv, 1e-10f);
-                const float dd = c - r;
-                phi += (c + 3.f * r) * dd * dd * dd * d->phi_factor;
-                const float g = r * dd * dd * d->dw_factor * rinv;
-                nx -= g * dx; ny -= g * dy; nz -= g * dz;
+            // the row in chunks of eight entries: the chunk's eight masks in
+            // flight, then the positions of its wall neighbors, the next
+            // chunk's entries loading meanwhile; terms added in row order
+            const int nch = (cnt + NBR_CHUNK - 1) / NBR_CHUNK;
+            int4 ea = make_int4(0, 0, 0, 0), eb = ea;
+            if (nch > 0) {
+                ea = row4[0];
+                eb = row4[1];
+            }
+            for (int ch = 0; ch < nch; ++ch) {
+                const uint32_t e[NBR_CHUNK] = {(uint32_t)ea.x, (uint32_t)ea.y, (uint32_t)ea.z, (uint32_t)ea.w,
+                                               (uint32_t)eb.x, (uint32_t)eb.y, (uint32_t)eb.z, (uint32_t)eb.w};
+                if (ch + 1 < nch) {
+                    ea = row4[(size_t)(ch + 1) * (NBR_CHUNK * WAVE / 4)];
+                    eb = row4[(size_t)(ch + 1) * (NBR_CHUNK * WAVE / 4) + 1];
+                }
+                bool wall[NBR_CHUNK];
+#pragma unroll
+                for (int k = 0; k < NBR_CHUNK; ++k) {
+                    const bool in = ch * NBR_CHUNK + k < cnt;
+                    wall[k] = in && (mask[(in ? e[k] : (uint32_t)i) & IDX_MASK] & wbit);
+                }
+                float4 q[NBR_CHUNK];
+#pragma unroll
+                for (int k = 0; k < NBR_CHUNK; ++k)
+                    if (wall[k]) q[k] = pos[e[k] & IDX_MASK];
+#pragma unroll
+                for (int k = 0; k < NBR_CHUNK; ++k) {
+                    if (!wall[k]) continue;
+                    float dx, dy, dz;
+                    pair_delta(p, q[k], e[k] >> IMG_SHIFT, d, dx, dy, dz);
+                    const float rsq = dx * dx + dy * dy + dz * dz;
+                    if (!(rsq < c2)) continue;
+                    const float rinv = __builtin_amdgcn_rsqf(rsq);
+                    const float r = fmaxf(rsq * rinv, 1e-10f);
+                    const float dd = c - r;
+                    phi += (c + 3.f * r) * dd * dd * dd * d->phi_factor;
+                    const float g = r * dd * dd * d->dw_factor * rinv;
+                    nx -= g * dx; ny -= g * dy; nz -= g * dz;
+                }
             }
             P.phi[sa][i] = phi;
             if (phi > d->phi_c) {

@@ -113,8 +113,8 @@ inline void type_bounds(const std::string &s, int nmax, int &lo, int &hi)
     if (lo < 1 || hi > nmax || lo > hi) fail(MDPD_E_ARG, "numeric type range '%s' out of 1..%d", s.c_str(), nmax);
 }
 
-enum Phase { PH_REBUILD = 0, PH_DENSITY, PH_FORCE, PH_INITIAL, PH_COUNT };
-const char *const PHASE_NAMES[PH_COUNT] = {"rebuild", "density", "force", "initial"};
+enum Phase { PH_REBUILD = 0, PH_DENSITY, PH_FORCE, PH_INITIAL, PH_WALL, PH_COUNT };
+const char *const PHASE_NAMES[PH_COUNT] = {"rebuild", "density", "force", "initial", "wall"};
 
 struct FixDesc {
     std::string id, style;
@@ -1729,7 +1729,7 @@ struct mdpd_system {
         timed(PH_FORCE, [&]() { hip_check(launch_force(P, pair_style, mode, ntypes == 1, split, stream), "force"); });
         if (t >= 0 && !obs.empty()) obs_post(t);
         if ((mode == MODE_FUSED || mode == MODE_FUSED_OUT) && hsys.fix_arb)
-            hip_check(launch_aboundary(P, 0, stream), "aboundary");          // post_integrate
+            timed(PH_WALL, [&]() { hip_check(launch_aboundary(P, 0, stream), "aboundary"); });   // post_integrate
         if (mode != MODE_SETUP)
             hip_check(launch_commit_step(P, mode == MODE_FINAL ? COMMIT_FINAL : COMMIT_DECIDE, split, stream), "commit");
     }
@@ -1799,7 +1799,7 @@ struct mdpd_system {
         if (nsteps == 0) return;
         PoolDev P = view();
         timed(PH_INITIAL, [&]() { hip_check(launch_initial(P, stream), "initial"); });
-        if (hsys.fix_arb) hip_check(launch_aboundary(P, 0, stream), "aboundary");   // post_integrate
+        if (hsys.fix_arb) timed(PH_WALL, [&]() { hip_check(launch_aboundary(P, 0, stream), "aboundary"); });
         hip_check(launch_commit_step(P, COMMIT_DECIDE, split, stream), "commit");
         const long long t0 = ntimestep;
         long long k = 1;                  // the step whose force pass runs next
