@@ -172,7 +172,7 @@ struct PoolDev {
     int *cell_sys;       // system of each cell slot
     int4 *cell_work;     // pools: the cells holding particles of the systems re-sorted this pass, as
                          // (system, packed coordinates, first particle, end) (k_scan_apply appends)
-    int *work_count;     // [0]: entries of cell_work (k_commit_rebuild clears it)
+    int *work_count;     // [0]: entries of cell_work (k_keys clears it)
     int2 *cell_lse;      // pools: particle range [start, end) of each cell by linear cell id (cell_base + lin)
     int *nlist;         // (cap/64) tiles * maxnbr * 64
     int *ncount;         // per particle
@@ -182,6 +182,8 @@ struct PoolDev {
     int *moved;          // per 64-particle tile: a particle moved past skin/2 this step
     int *status;         // pooled engine, per system: job << 3 | 1 stale | 2 done | 4 overflow
     int *cl;             // per particle slot, in cell-segment order: sub-bin key (re-sort scratch)
+    int *wall_near;      // per particle: its neighbor row holds a wall particle (wall branch; set by
+                         // the first wall pass after each build, valid until the next)
     SysDev *sys;
     float *coef;         // pooled coefficient tables
     float *minv;         // pooled per-type 1/m

@@ -627,9 +627,14 @@ __global__ void __launch_bounds__(WAVE) k_aboundary(PoolDev P, int ntiles, int p
     int moved = 0;
     if (valid) {
         float4 p = pos[i];
-        if (mask[i] & d->fix_mobile_bit) {
+        // the rows do not change between builds: the first wall pass after a
+        // build records which of them hold a wall particle, and the later
+        // passes skip the others (no wall neighbor: phi stays 0, no bounce)
+        const bool fresh = d->ago == 0;
+        if ((mask[i] & d->fix_mobile_bit) && (fresh || P.wall_near[i])) {
             const int maxnbr = P.maxnbr;
             const int cnt = min(P.ncount[i], maxnbr);
+            int near = 0;
             const int4 *row4 = (const int4 *)(P.nlist + nbr_at(i, 0, maxnbr));
             const float c = d->cut_phi, c2 = c * c;
             const int wbit = d->fix_wall_bit;
@@ -655,6 +660,7 @@ __global__ void __launch_bounds__(WAVE) k_aboundary(PoolDev P, int ntiles, int p
                 for (int k = 0; k < NBR_CHUNK; ++k) {
                     const bool in = ch * NBR_CHUNK + k < cnt;
                     wall[k] = in && (mask[(in ? e[k] : (uint32_t)i) & IDX_MASK] & wbit);
+                    near |= wall[k] ? 1 : 0;
                 }
                 float4 q[NBR_CHUNK];
 #pragma unroll
@@ -675,6 +681,7 @@ __global__ void __launch_bounds__(WAVE) k_aboundary(PoolDev P, int ntiles, int p
                     nx -= g * dx; ny -= g * dy; nz -= g * dz;
                 }
             }
+            if (fresh) P.wall_near[i] = near;
             P.phi[sa][i] = phi;
             if (phi > d->phi_c) {
                 const float inv = __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);

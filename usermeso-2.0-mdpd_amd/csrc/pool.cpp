@@ -202,7 +202,7 @@ struct mdpd_pool {
     DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys, work_count;
     DevBuf<int4> cell_work;
     DevBuf<int2> cell_lse;
-    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl;
+    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl, wall_near;
     DevBuf<SysDev> dsys;
     RangeAlloc parts, cells;              // parts in units of 256 slots
     std::vector<int> free_slots, slot_job;
@@ -327,7 +327,7 @@ struct mdpd_pool {
         P.work_count = work_count.p; P.cell_work = cell_work.p; P.cell_lse = cell_lse.p;
         P.nlist = nlist.p; P.ncount = ncount.p; P.wave_sys = wave_sys.p;
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
-        P.cl = cl.p;
+        P.cl = cl.p; P.wall_near = wall_near.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
         P.cap = cap; P.nact = nact; P.ncellslots = nslots; P.ncact = ncact; P.nsys = nsys_act; P.maxsub = maxsub;
         P.maxnbr = maxnbr;
@@ -421,7 +421,7 @@ struct mdpd_pool {
         nlist.alloc((size_t)cap * maxnbr);
         clist.alloc((size_t)cap * maxnbr);
         wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE); status.alloc(max_sys);
-        cl.alloc(cap);
+        cl.alloc(cap); wall_near.alloc(cap);
         cell_count.alloc((size_t)nslots * SUBCELLS); cell_start.alloc((size_t)nslots * SUBCELLS);
         cell_xyz.alloc(nslots);
         cell_rank.alloc(nslots); cell_sys.alloc(nslots);

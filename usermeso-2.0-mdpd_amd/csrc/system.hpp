@@ -223,7 +223,7 @@ struct mdpd_system {
     DevBuf<int> pcell, pslot, psub, tmp, cell_count, cell_start, cell_xyz, cell_rank, cell_sys, work_count;
     DevBuf<int4> cell_work;
     DevBuf<int2> cell_lse;
-    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl;
+    DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl, wall_near;
     DevBuf<unsigned long long> key;
     DevBuf<SysDev> dsys;
     int *pinned = nullptr;            // host-visible readback of flags
@@ -1387,7 +1387,7 @@ struct mdpd_system {
         P.work_count = work_count.p; P.cell_work = cell_work.p; P.cell_lse = cell_lse.p;
         P.nlist = nlist.p; P.ncount = ncount.p; P.wave_sys = wave_sys.p;
         P.clist = clist.p; P.ccount = ccount.p; P.moved = moved.p; P.status = status.p;
-        P.cl = cl.p;
+        P.cl = cl.p; P.wall_near = wall_near.p;
         P.sys = dsys.p; P.coef = coef.p; P.minv = minv.p; P.mass = mmass.p;
         P.cap = cap; P.nact = cap; P.ncellslots = nslots; P.ncact = nslots; P.nsys = 1; P.maxsub = 0;
         P.maxnbr = maxnbr;
@@ -1579,7 +1579,7 @@ struct mdpd_system {
         nlist.alloc((size_t)cap * maxnbr);
         clist.alloc((size_t)cap * maxnbr);
         wave_sys.alloc(cap / WAVE); moved.alloc(cap / WAVE); status.alloc(1);
-        cl.alloc(cap);
+        cl.alloc(cap); wall_near.alloc(cap);
         cell_count.alloc((size_t)nslots * SUBCELLS); cell_start.alloc((size_t)nslots * SUBCELLS);
         cell_xyz.alloc(nslots);
         cell_rank.alloc(nslots); cell_sys.alloc(nslots);
