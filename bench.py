@@ -620,6 +620,9 @@ def run_headline(a, legs):
     if "wall" in legs:
         log("wall")
         config["wall"] = wall_record(a.job_steps)
+    if "obs" in legs:
+        log("observables")
+        config["observables"] = observables_record(a.job_steps)
     if "cpu" in legs:
         med = int(np.argsort(sizes)[len(sizes) // 2])
         box, x, v, _ = specs[med]
@@ -815,6 +818,43 @@ def wall_record(steps, profile_steps=200, split=None):
     }
 
 
+def observables_record(steps):
+    """The device observables (fix rdf/fast/meso every 10 steps, fix
+    density/meso every 100 steps over a 10-step window) on the 96,800-particle
+    slab: the same run with and without them."""
+    import tempfile
+    w = SLABS["slab-literal"]
+    box = w["box"]
+    x, v = make_slab(box, w["slab"])
+    n = len(x)
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for tag, extra in (("without", ""),
+                           ("with", f"fix g all rdf/fast/meso output {tmp}/rdf.out nbin 100 every 10\n"
+                                    f"fix p all density/meso output {tmp}/prof along z nbin 40 every 100 window 10")):
+            best = None
+            for rep in range(2):
+                s = system(box, x, v, 4242, lucy=True)
+                s.tune("split", 4)
+                if extra:
+                    s.commands(extra)
+                s.setup()
+                t0 = time.perf_counter()
+                s.run(steps)
+                dt = time.perf_counter() - t0
+                s.close()
+                best = dt if best is None else min(best, dt)
+            out[tag] = best
+    return {
+        "workload": f"plain mDPD liquid slab 40x40x20 (N={n}, Lucy weight, split 4), {steps} steps, with and "
+                    f"without fix rdf/fast/meso (100 bins, every 10 steps) and fix density/meso (40 bins along z, "
+                    f"every 100 steps, 10-step window)",
+        "particles": n, "steps": steps,
+        "value_without": n * steps / out["without"], "value_with": n * steps / out["with"],
+        "overhead_us_per_step": (out["with"] - out["without"]) / steps * 1e6,
+    }
+
+
 def run_wall(a):
     r = wall_record(a.job_steps)
     log(f"wall: {r['value']:.3e} particle-steps/s, wall pass {r['wall_pass_us']:.1f} us")
@@ -909,8 +949,8 @@ def main():
     ap.add_argument("--job-steps", type=int, default=2000, help="steps of every batched / served job")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "batched", "served", "pore", "dpd", "wall"] + sorted(SLABS))
-    ap.add_argument("--legs", default="batched,slab,wall,cpu,served",
-                    help="legs of the headline workload (batched, slab, wall, cpu, served)")
+    ap.add_argument("--legs", default="batched,slab,wall,obs,cpu,served",
+                    help="legs of the headline workload (batched, slab, wall, obs, cpu, served)")
     ap.add_argument("--jobs", type=int, default=256)
     ap.add_argument("--seed", type=int, default=20241202)
     ap.add_argument("--budget-gb", type=float, default=16.0, help="HBM budget of the batched pool")

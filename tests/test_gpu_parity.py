@@ -513,6 +513,29 @@ def _read_dump(path):
     return frames
 
 
+def test_observables_between_replayed_steps(engine, tmp_path):
+    # with observable fixes active, the steps on which none of them samples
+    # or writes are replayed from the captured step graph: samples, files and
+    # trajectory equal a run that launches every step's kernels
+    n, L = 1296, 6.0
+    x, v = uniform_box(n, L, 23)
+    res = []
+    for graph in (1, 0):
+        d = tmp_path / f"graph{graph}"
+        d.mkdir()
+        s = mdpd_system(engine, x, v, L, extra=f"""
+            fix g all rdf/fast/meso output {d}/rdf.out nbin 20 every 7
+            fix p all density/meso output {d}/prof along z nbin 6 every 50 window 3
+        """)
+        s.tune("graph", graph)
+        s.run(120)
+        res.append((s.fix_result("g")[1], s.fix_result("p")[1], s.gather("x"), s.gather("v"),
+                    np.loadtxt(d / "prof.000000100")))
+        s.close()
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_dump_custom_frames_match_the_state_of_their_step(engine, tmp_path):
     n, L = 1296, 6.0
     x, v = uniform_box(n, L, 31)

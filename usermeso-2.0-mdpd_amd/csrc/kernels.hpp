@@ -53,7 +53,8 @@ constexpr size_t HARVEST_SLOT_BYTES = 3 * 16 + 4 * 4 + 4 + 6 * 4;
 inline size_t harvest_bytes(size_t npad) { return npad * HARVEST_SLOT_BYTES + sizeof(SysDev); }
 hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, void *dst, size_t room, hipStream_t st);
 
-// observables (accumulate into device histograms)
+// observables (accumulate into device histograms of nbin + 1 slots, slot nbin
+// counting the samples; a system whose rows overflowed is not sampled)
 hipError_t launch_rdf(const PoolDev &P, unsigned *hist, int nbin, float rc, int gi, int gj, hipStream_t st);
 hipError_t launch_profile(const PoolDev &P, unsigned *count, int nbin, int axis, int g, hipStream_t st);
 

@@ -2,7 +2,9 @@
 // (fix rdf/fast/meso) and binned number density along an axis
 // (fix density/meso).  Both count into LDS per workgroup and add the
 // workgroup's bins to the global histogram once (one atomic per bin per
-// workgroup).
+// workgroup).  Slot nbin of a histogram counts the samples taken; a step
+// whose rows overflowed (the step is repeated once they are widened) is
+// not sampled.  One system (the plain path).
 //
 // Replaces:
 //   MesoFixRDFFast::post_force / gpu_calc_rdf        fix_rdf_fast_meso.cu:102-175
@@ -20,6 +22,8 @@ constexpr int OBS_MAXBIN = 4096;      // LDS bins per workgroup
 // pair appears from both sides, as with the reference's full list.
 __global__ void __launch_bounds__(OBS_BLK) k_rdf(PoolDev P, unsigned *hist, int nbin, float rc, int gi, int gj)
 {
+    if (P.sys->overflow) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) hist[nbin] += 1u;
     __shared__ unsigned h[OBS_MAXBIN];
     for (int b = threadIdx.x; b < nbin; b += OBS_BLK) h[b] = 0u;
     __syncthreads();
@@ -68,6 +72,8 @@ __global__ void __launch_bounds__(OBS_BLK) k_rdf(PoolDev P, unsigned *hist, int 
 // past the box between re-sorts) are not counted
 __global__ void __launch_bounds__(OBS_BLK) k_profile(PoolDev P, unsigned *count, int nbin, int axis, int g)
 {
+    if (P.sys->overflow) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) count[nbin] += 1u;
     __shared__ unsigned h[OBS_MAXBIN];
     for (int b = threadIdx.x; b < nbin; b += OBS_BLK) h[b] = 0u;
     __syncthreads();

@@ -142,8 +142,8 @@ struct DumpDesc {
 
 // device histogram and sample counts of an observable fix
 struct ObsState {
-    DevBuf<unsigned> hist;
-    long long samples = 0;       // rdf: sampled steps; density: frames of the current window
+    DevBuf<unsigned> hist;       // nbin counts, then the samples taken (rdf: sampled steps;
+                                 // density: frames of the current window)
     long long last_dump = -1;
 };
 
@@ -241,12 +241,14 @@ struct mdpd_system {
     // reads the device step counter once per batch of STEP_BATCH steps ----
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
+    std::map<uint32_t, hipGraphExec_t> gobs;   // steps with observable samples, by sample mask
     bool graph_valid = false;
     static constexpr int STEP_BATCH = 64;
 
     ~mdpd_system()
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto &v : gobs) (void)hipGraphExecDestroy(v.second);
         for (auto &f : fixes) {
             try { finish_obs(f); } catch (...) {}
         }
@@ -1189,6 +1191,7 @@ struct mdpd_system {
             obs.erase(f.id);
             ObsState &o = obs[f.id];
             (void)o;
+            graph_valid = false;         // the captured sampling steps change
             return;                      // observables do not change the step: no new setup
         }
         dirty = true;
@@ -1205,17 +1208,19 @@ struct mdpd_system {
         if (!o.hist.p) {
             ensure_stream();
             hip_check(hipSetDevice(device), "hipSetDevice");
-            o.hist.alloc(f.nbin);
-            hip_check(hipMemsetAsync(o.hist.p, 0, sizeof(unsigned) * f.nbin, stream), "memset");
+            o.hist.alloc(f.nbin + 1);
+            hip_check(hipMemsetAsync(o.hist.p, 0, sizeof(unsigned) * (f.nbin + 1), stream), "memset");
         }
         return o;
     }
 
+    // the nbin counts and the sample count
     std::vector<unsigned> obs_counts(ObsState &o, int nbin)
     {
-        std::vector<unsigned> h(nbin, 0u);
+        std::vector<unsigned> h(nbin + 1, 0u);
         if (o.hist.p) {
-            hip_check(hipMemcpyAsync(h.data(), o.hist.p, sizeof(unsigned) * nbin, hipMemcpyDeviceToHost, stream), "download");
+            hip_check(hipMemcpyAsync(h.data(), o.hist.p, sizeof(unsigned) * (nbin + 1), hipMemcpyDeviceToHost, stream),
+                      "download");
             hip_check(hipStreamSynchronize(stream), "sync");
         }
         return h;
@@ -1229,6 +1234,7 @@ struct mdpd_system {
     {
         ObsState &o = obs[f.id];
         std::vector<unsigned> h = obs_counts(o, f.nbin);
+        const long long samples = h[f.nbin];
         x.assign(f.nbin, 0.0);
         val.assign(f.nbin, 0.0);
         if (f.style == "rdf/fast/meso") {
@@ -1242,8 +1248,8 @@ struct mdpd_system {
             const double rho_j = nj / V;
             for (int b = 0; b < f.nbin; ++b) {
                 x[b] = (b + 0.5) * bin;
-                if (!o.samples || !ni || !nj) continue;
-                const double freq = h[b] / double(ni) / double(o.samples);
+                if (!samples || !ni || !nj) continue;
+                const double freq = h[b] / double(ni) / double(samples);
                 const double shell = 4. / 3. * 3.1415 * (std::pow(bin * (b + 1), 3.) - std::pow(bin * b, 3.));
                 val[b] = freq / shell / rho_j;
             }
@@ -1251,7 +1257,7 @@ struct mdpd_system {
             const double bin = (hi[f.along] - lo[f.along]) / f.nbin;
             for (int b = 0; b < f.nbin; ++b) {
                 x[b] = (b + 0.5) * bin;
-                val[b] = o.samples ? h[b] / double((float)o.samples) : 0.0;
+                val[b] = samples ? h[b] / double((float)samples) : 0.0;
             }
         }
     }
@@ -1266,34 +1272,71 @@ struct mdpd_system {
         fclose(fp);
     }
 
-    // per step t of a run: density/meso samples x(t) before the force pass
-    // (post_integrate), rdf/fast/meso after it (post_force)
-    void obs_pre(long long t)
+    // Observables per step t of a run: density/meso samples x(t) before the
+    // force pass (post_integrate), rdf/fast/meso after it (post_force); a
+    // density window is written on the step after it.  Bit k of a sample
+    // mask stands for the k-th observable fix (in definition order).
+    static bool density_sample(const FixDesc &f, long long t)
     {
-        for (auto &f : fixes) {
-            if (f.style != "density/meso") continue;
-            ObsState &o = obs_state(f);
-            const long long n = (t + f.every / 2) / f.every, m = n * f.every;
-            if (t - m >= -f.window && t - m < f.window) {
-                PoolDev P = view();
-                hip_check(launch_profile(P, o.hist.p, f.nbin, f.along, f.groupbit, stream), "density profile");
-                ++o.samples;
-            } else if (t - m == f.window) {
-                dump_profile(f, o, m, t);
-            }
-        }
+        const long long m = (t + f.every / 2) / f.every * f.every;
+        return t - m >= -f.window && t - m < f.window;
     }
-
-    void obs_post(long long t)
+    static bool density_dump(const FixDesc &f, long long t)
     {
+        const long long m = (t + f.every / 2) / f.every * f.every;
+        return t - m == f.window;
+    }
+    static bool obs_samples(const FixDesc &f, long long t)
+    {
+        return f.style == "rdf/fast/meso" ? t % f.every == 0 : density_sample(f, t);
+    }
+    uint32_t obs_sample_mask(long long t) const     // the first 32 fixes
+    {
+        uint32_t mask = 0;
+        int k = 0;
+        for (const auto &f : fixes) {
+            if (!is_obs(f)) continue;
+            if (k < 32 && obs_samples(f, t)) mask |= 1u << k;
+            ++k;
+        }
+        return mask;
+    }
+    // a step on which a density window is written (host work: the direct path)
+    bool obs_dump_due(long long t) const
+    {
+        for (const auto &f : fixes)
+            if (f.style == "density/meso" && density_dump(f, t)) return true;
+        return false;
+    }
+    // sampling launches of the fixes that sample step t (t < 0: the fixes
+    // in `mask`), the density ones (pre) or the rdf ones (post); the kernels
+    // count their samples on the device
+    void obs_launch(long long t, uint32_t mask, bool pre)
+    {
+        int k = 0;
         for (auto &f : fixes) {
-            if (f.style != "rdf/fast/meso" || t % f.every != 0) continue;
+            if (!is_obs(f)) continue;
+            const bool on = t >= 0 ? obs_samples(f, t) : k < 32 && ((mask >> k) & 1u);
+            ++k;
+            if (!on || (f.style == "density/meso") != pre) continue;
             ObsState &o = obs_state(f);
             PoolDev P = view();
-            hip_check(launch_rdf(P, o.hist.p, f.nbin, (float)max_cut(), f.groupbit, f.other_bit, stream), "rdf");
-            ++o.samples;
+            if (pre)
+                hip_check(launch_profile(P, o.hist.p, f.nbin, f.along, f.groupbit, stream), "density profile");
+            else
+                hip_check(launch_rdf(P, o.hist.p, f.nbin, (float)max_cut(), f.groupbit, f.other_bit, stream), "rdf");
         }
     }
+    void obs_pre(long long t)
+    {
+        for (auto &f : fixes)
+            if (f.style == "density/meso" && density_dump(f, t)) {
+                const long long m = (t + f.every / 2) / f.every * f.every;
+                dump_profile(f, obs_state(f), m, t);
+            }
+        obs_launch(t, 0, true);
+    }
+    void obs_post(long long t) { obs_launch(t, 0, false); }
 
     void dump_profile(const FixDesc &f, ObsState &o, long long m, long long t)
     {
@@ -1301,8 +1344,7 @@ struct mdpd_system {
         snprintf(fn, sizeof(fn), "%s.%09lld", f.output.c_str(), m);
         write_obs(f, fn);
         o.last_dump = t;
-        o.samples = 0;
-        hip_check(hipMemsetAsync(o.hist.p, 0, sizeof(unsigned) * f.nbin, stream), "memset");
+        hip_check(hipMemsetAsync(o.hist.p, 0, sizeof(unsigned) * (f.nbin + 1), stream), "memset");
     }
 
     // end of an observable fix (unfix, redefinition, handle closed): the
@@ -1316,10 +1358,11 @@ struct mdpd_system {
         if (f.style == "rdf/fast/meso") {
             write_obs(f, f.output);
         } else {
-            const long long t = ntimestep, n = (t + f.every / 2) / f.every, m = n * f.every;
-            if (t - m >= -f.window && t - m < f.window && o.last_dump < t && o.samples) dump_profile(f, o, m, t);
+            const long long t = ntimestep, m = (t + f.every / 2) / f.every * f.every;
+            if (density_sample(f, t) && o.last_dump < t && obs_counts(o, f.nbin)[f.nbin]) dump_profile(f, o, m, t);
         }
         obs.erase(it);
+        graph_valid = false;
     }
 
     // ------------------------------------------------------------------
@@ -1720,14 +1763,17 @@ struct mdpd_system {
     }
 
     // t: timestep this pass evaluates (observables sample it), -1 at setup
-    void force_pass(int mode, long long t = -1)
+    // and in a captured step, which takes the samples of `mask` instead
+    void force_pass(int mode, long long t = -1, uint32_t mask = 0)
     {
         PoolDev P = view();
         const bool with_rho = pair_style == STYLE_MDPD && hsys.rho_bit;
         if (t >= 0 && !obs.empty()) obs_pre(t);
+        else if (mask) obs_launch(-1, mask, true);
         timed(PH_DENSITY, [&]() { hip_check(launch_first(P, with_rho, split, stream), "first pass"); });
         timed(PH_FORCE, [&]() { hip_check(launch_force(P, pair_style, mode, ntypes == 1, split, stream), "force"); });
         if (t >= 0 && !obs.empty()) obs_post(t);
+        else if (mask) obs_launch(-1, mask, false);
         if ((mode == MODE_FUSED || mode == MODE_FUSED_OUT) && hsys.fix_arb)
             timed(PH_WALL, [&]() { hip_check(launch_aboundary(P, 0, stream), "aboundary"); });   // post_integrate
         if (mode != MODE_SETUP)
@@ -1744,28 +1790,57 @@ struct mdpd_system {
         hip_check(launch_commit_rebuild(P, stream), "commit rebuild");
     }
 
-    // one fused step (rebuild phase, passes, commit) as a hipGraph
-    void capture_step()
+    // the graph of one fused step whose observables take the samples of
+    // `mask` (0: none); all are captured again after a change
+    hipGraphExec_t step_graph(uint32_t mask)
     {
-        if (gexec) {
-            hip_check(hipGraphExecDestroy(gexec), "hipGraphExecDestroy");
+        if (!graph_valid) {
+            if (gexec) hip_check(hipGraphExecDestroy(gexec), "hipGraphExecDestroy");
             gexec = nullptr;
+            for (auto &v : gobs) hip_check(hipGraphExecDestroy(v.second), "hipGraphExecDestroy");
+            gobs.clear();
+            gexec = capture_step(0);
+            graph_valid = true;
         }
+        if (!mask) return gexec;
+        auto it = gobs.find(mask);
+        if (it != gobs.end()) return it->second;
+        for (auto &f : fixes)
+            if (is_obs(f)) obs_state(f);           // histograms allocated outside the capture
+        return gobs[mask] = capture_step(mask);
+    }
+
+    // one fused step (rebuild phase, passes, commit) as a hipGraph
+    hipGraphExec_t capture_step(uint32_t mask)
+    {
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
             rebuild_phase();
-            force_pass(MODE_FUSED);
+            force_pass(MODE_FUSED, -1, mask);
         } catch (...) {
             (void)hipStreamEndCapture(stream, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
         hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
-        const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        hipGraphExec_t x = nullptr;
+        const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         hip_check(e, "hipGraphInstantiate");
-        graph_valid = true;
+        return x;
+    }
+
+    // a step the graph does not replay because of its observables: a density
+    // window is written, more than 32 observable fixes, or a sample pattern
+    // beyond the OBS_VARIANTS graphs kept
+    static constexpr size_t OBS_VARIANTS = 16;
+    bool obs_direct(long long t) const
+    {
+        if (obs.empty()) return false;
+        if (obs.size() > 32 || obs_dump_due(t)) return true;
+        const uint32_t mask = obs_sample_mask(t);
+        return mask && graph_valid && !gobs.count(mask) && gobs.size() >= OBS_VARIANTS;
     }
 
     // Drain the stream and read the device step counter; rows that
@@ -1806,7 +1881,7 @@ struct mdpd_system {
         while (k < nsteps) {
             const long long t = t0 + k;
             const bool out = !dumps.empty() && dump_due(t);
-            if (out || !obs.empty() || prof || !use_graph) {
+            if (out || obs_direct(t) || prof || !use_graph) {
                 // a step with host work: its rebuild settles before the passes
                 rebuild_phase();
                 if (settle() != t) fail(MDPD_E_HIP, "device step counter out of order at step %lld", t);
@@ -1816,9 +1891,12 @@ struct mdpd_system {
                 continue;
             }
             long long m = 1;
-            while (m < STEP_BATCH && k + m < nsteps && !(!dumps.empty() && dump_due(t + m))) ++m;
-            if (!graph_valid) capture_step();
-            for (long long q = 0; q < m; ++q) hip_check(hipGraphLaunch(gexec, stream), "hipGraphLaunch");
+            while (m < STEP_BATCH && k + m < nsteps && !(!dumps.empty() && dump_due(t + m)) && !obs_direct(t + m))
+                ++m;
+            // step t+q replays the graph whose observables sample t+q; a step
+            // that overflows skips the rest of the batch, samples included
+            for (long long q = 0; q < m; ++q)
+                hip_check(hipGraphLaunch(step_graph(obs.empty() ? 0 : obs_sample_mask(t + q)), stream), "hipGraphLaunch");
             k = settle() - t0;
             if (k < 1 || k > nsteps) fail(MDPD_E_HIP, "device step counter out of range (%lld)", k + t0);
         }
