@@ -536,6 +536,32 @@ def test_observables_between_replayed_steps(engine, tmp_path):
         np.testing.assert_array_equal(a, b)
 
 
+def test_observables_beyond_the_kept_step_graphs(engine, tmp_path):
+    # five rdf fixes sampling every 2, 3, 5, 7 and 11 steps give more sample
+    # patterns than step graphs are kept for, so some sampling steps launch
+    # their kernels directly; an unfix between two runs drops the captured
+    # graphs.  Everything equals a run that launches every step's kernels.
+    n, L = 1296, 6.0
+    x, v = uniform_box(n, L, 29)
+    every = (2, 3, 5, 7, 11)
+    res = []
+    for graph in (1, 0):
+        d = tmp_path / f"graph{graph}"
+        d.mkdir()
+        fixes = "\n".join(f"fix g{e} all rdf/fast/meso output {d}/rdf{e}.out nbin 16 every {e}" for e in every)
+        s = mdpd_system(engine, x, v, L, extra=fixes + f"\nfix p all density/meso output {d}/prof along y nbin 5 every 20 window 2")
+        s.tune("graph", graph)
+        s.run(70)
+        s.command("unfix g3")
+        s.run(80)
+        res.append([s.fix_result(f"g{e}")[1] for e in every if e != 3] +
+                   [s.fix_result("p")[1], s.gather("x"), s.gather("v"), np.loadtxt(d / "prof.000000140")])
+        s.close()
+        res[-1].append(np.loadtxt(d / "rdf3.out"))
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_dump_custom_frames_match_the_state_of_their_step(engine, tmp_path):
     n, L = 1296, 6.0
     x, v = uniform_box(n, L, 31)
