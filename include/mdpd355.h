@@ -55,7 +55,9 @@ const char *mdpd_version(void);
  *   [cut_phi c phi_c p rho_wall w wall G mobile G],
  * pair_style mdpd/meso T rc seed <mobile group> <wall group>,
  * fix ID G addforce/meso fx fy fz, fix ID G rdf/fast/meso output F nbin N
- * [every M] [other G], fix ID G density/meso output F along x|y|z nbin N
+ * [every M] [other G] [pi 3.1415|exact] (pi: an option of this library, the
+ * constant of the shell volumes; 3.1415, the reference's, by default),
+ * fix ID G density/meso output F along x|y|z nbin N
  * every M window W, read_data F, write_data F, write_restart F,
  * read_restart F (checkpoint: every atom as the record of
  * AtomVecMDPDAtomic::pack_restart, atom_vec_mdpd_atomic_meso.cu:548-570, in

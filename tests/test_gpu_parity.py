@@ -480,11 +480,16 @@ def test_observable_fixes_count_what_the_host_counts(engine, tmp_path):
     x, v = uniform_box(n, L, 21)
     s = mdpd_system(engine, x, v, L, extra=f"""
         fix g all rdf/fast/meso output {tmp_path}/rdf.out nbin 20 every 5
+        fix ge all rdf/fast/meso output {tmp_path}/rdf_pi.out nbin 20 every 5 pi exact
         fix p all density/meso output {tmp_path}/prof along z nbin 6 every 1000 window 10
     """)
     s.run(5)                       # rdf samples step 5; density samples steps 1..5 (window [-10, 10) of m = 0)
     xr = s.gather("x")
     _, g = s.fix_result("g")
+    # the same counts over shells computed with pi
+    np.testing.assert_allclose(s.fix_result("ge")[1], g * 3.1415 / np.pi, rtol=1e-12)
+    with pytest.raises(RuntimeError, match="pi must be"):
+        s.command(f"fix gx all rdf/fast/meso output {tmp_path}/x.out nbin 20 pi 3.14")
     d = xr[:, None, :] - xr[None, :, :]
     d -= L * np.round(d / L)
     r = np.sqrt((d * d).sum(-1))[~np.eye(n, dtype=bool)]

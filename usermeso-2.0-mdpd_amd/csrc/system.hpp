@@ -129,6 +129,7 @@ struct FixDesc {
     // (fix_density_profile_meso.cpp:30-83)
     std::string output;
     int nbin = 0, every = 1, window = 0, along = -1, other_bit = 0;
+    int exact_pi = 0;    // rdf: shell volumes with pi instead of the reference's 3.1415
 };
 
 // dump ID group custom N file columns... (src/dump_custom.cpp)
@@ -1146,7 +1147,9 @@ struct mdpd_system {
             if (f.arb && (f.rd == 0 || f.cut_phi == 0 || f.phi_c == 0 || f.rho_wall == 0))
                 fail(MDPD_E_ARG, "<MESO> Fix aboundary/rho/meso parameters are not set properly.");
         } else if (f.style == "rdf/fast/meso") {
-            // MesoFixRDFFast ctor: output FILE nbin N [every M] [other G]
+            // MesoFixRDFFast ctor: output FILE nbin N [every M] [other G];
+            // [pi 3.1415|exact] chooses the shell-volume constant (an option
+            // of this library: the reference always uses 3.1415)
             for (size_t k = 4; k < w.size(); ++k) {
                 if (w[k] == "output" && k + 1 < w.size()) f.output = w[++k];
                 else if (w[k] == "nbin" && k + 1 < w.size()) f.nbin = (int)parse_int(w[++k]);
@@ -1154,6 +1157,10 @@ struct mdpd_system {
                 else if (w[k] == "other" && k + 1 < w.size()) {
                     f.other_bit = group_bit(w[++k]);
                     if (!f.other_bit) fail(MDPD_E_ARG, "<MESO> Undefined other group id in fix rdf/meso");
+                } else if (w[k] == "pi" && k + 1 < w.size()) {
+                    const std::string &v = w[++k];
+                    if (v != "3.1415" && v != "exact") fail(MDPD_E_ARG, "fix rdf/fast/meso: pi must be 3.1415 or exact");
+                    f.exact_pi = v == "exact";
                 }
             }
             if (f.output.empty() || f.nbin <= 0) fail(MDPD_E_ARG, "Incomplete compute rdf command: insufficient arguments");
@@ -1250,7 +1257,8 @@ struct mdpd_system {
                 x[b] = (b + 0.5) * bin;
                 if (!samples || !ni || !nj) continue;
                 const double freq = h[b] / double(ni) / double(samples);
-                const double shell = 4. / 3. * 3.1415 * (std::pow(bin * (b + 1), 3.) - std::pow(bin * b, 3.));
+                const double pi = f.exact_pi ? 3.14159265358979323846 : 3.1415;
+                const double shell = 4. / 3. * pi * (std::pow(bin * (b + 1), 3.) - std::pow(bin * b, 3.));
                 val[b] = freq / shell / rho_j;
             }
         } else {
