@@ -237,6 +237,17 @@ int mdpd_pool_done(mdpd_pool *pool, int *jobs, int max, int *count);
 int mdpd_pool_gather(mdpd_pool *pool, int job, const char *name, int type, int count, void *data);
 /* {T, P, pe, ke} of a finished job, as mdpd_thermo. */
 int mdpd_pool_thermo(mdpd_pool *pool, int job, double out[4]);
+
+/* Result of a finished job's fix rdf/fast/meso, as mdpd_fix_result returns
+ * it for a handle that ran the job (same bins, counts and normalization).
+ * No counterpart in the reference (LAMMPS runs one system); the fix samples
+ * each pooled job after its force pass of every step that is a multiple of
+ * `every` (fix_rdf_fast_meso.cu:140-175).  At most POOL_RDF = 2 such fixes
+ * per job; the output file is not written for pool jobs, and fix
+ * density/meso is not available for them.  The histograms take 32 KiB of
+ * device memory per resident-job slot, outside the HBM budget, from the
+ * first job that has the fix. */
+int mdpd_pool_fix_result(mdpd_pool *pool, int job, const char *fix_id, int n, double *x, double *value, int *nbin);
 /* Atoms, final timestep, and submit / admit / done times (seconds on
  * mdpd_clock) of a job. */
 int mdpd_pool_job_info(mdpd_pool *pool, int job, long long *natoms, long long *timestep, double times[3]);

@@ -319,3 +319,45 @@ def test_split_change_before_a_harvest_keeps_the_last_pass(engine):
     got = ({k: p.gather(ja, k) for k in ("x", "v", "f", "image", "e_pair", "virial")}, p.thermo(ja),
            p.info(ja)["timestep"])
     assert_same(got, alone, "job harvested after a split change")
+
+
+def test_rdf_of_pool_jobs_equals_plain_runs(engine, tmp_path):
+    # rdf/fast/meso on pooled jobs: each job's g(r) equals its handle's after
+    # the same run, bit for bit (two fixes on one job: every 5 over all
+    # pairs, every 3 over pairs with a type-2 partner and pi exact; one fix
+    # sampling every step on a second job; a third job without the fix)
+    def fixes(tag):
+        if tag == "a":
+            return (f"group b type 2\n"
+                    f"fix g1 all rdf/fast/meso output {tmp_path}/a1.out nbin 25 every 5\n"
+                    f"fix g2 all rdf/fast/meso output {tmp_path}/a2.out nbin 10 every 3 other b pi exact")
+        if tag == "b":
+            return f"fix g all rdf/fast/meso output {tmp_path}/b.out nbin 40"
+        return ""
+    specs = [("a", 2000, 71, 2, 23), ("b", 3000, 72, 1, 17), ("c", 2500, 73, 1, 30)]
+    make = lambda t, n, seed, nt: droplet(engine, n, seed, ntypes=nt, extra=fixes(t))
+    ids = {"a": ("g1", "g2"), "b": ("g",), "c": ()}
+    want = {}
+    for t, n, seed, nt, steps in specs:
+        s = make(t, n, seed, nt)
+        s.tune("split", 2)
+        s.run(steps)
+        want[t] = [s.fix_result(f) for f in ids[t]]
+        s.close()
+    got, p = pool_results(engine, [make(t, n, seed, nt) for t, n, seed, nt, _ in specs],
+                          [steps for *_, steps in specs])
+    for k, (t, *_rest) in enumerate(specs):
+        job = k + 1
+        for f, (x, g) in zip(ids[t], want[t]):
+            xp, gp = p.fix_result(job, f)
+            np.testing.assert_array_equal(xp, x, err_msg=f"{t}:{f}")
+            np.testing.assert_array_equal(gp, g, err_msg=f"{t}:{f}")
+            assert g.max() > 0, (t, f)
+    with pytest.raises(RuntimeError, match="no rdf"):
+        p.fix_result(3, "g")
+    # density profiles and a third rdf fix are refused at submission
+    with pytest.raises(RuntimeError, match="not available for pool jobs"):
+        p.submit(droplet(engine, 2000, 74, extra=f"fix p all density/meso output {tmp_path}/p along z nbin 4 every 10 window 2"), 5)
+    three = "\n".join(f"fix r{k} all rdf/fast/meso output {tmp_path}/r{k} nbin 8" for k in range(3))
+    with pytest.raises(RuntimeError, match="at most 2"):
+        p.submit(droplet(engine, 2000, 75, extra=three), 5)

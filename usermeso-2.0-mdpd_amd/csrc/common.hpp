@@ -10,6 +10,8 @@ namespace mdpd {
 
 constexpr int WAVE = 64;
 constexpr int MAXTYPES = 8;
+constexpr int POOL_RDF = 2;        // rdf/fast/meso fixes per pooled job
+constexpr int OBS_MAXBIN = 4096;   // bins of an observable histogram
 constexpr int NCOEF = 10;          // per type pair, see Coef
 constexpr int IMG_SHIFT = 27;      // neighbor entry = j | image << 27
 constexpr uint32_t IDX_MASK = (1u << IMG_SHIFT) - 1u;
@@ -140,6 +142,13 @@ struct SysDev {
     // (fix aboundary/rho/meso ... wall G mobile G)
     int pair_arb, pair_mobile_bit, pair_wall_bit;
     int fix_arb, fix_mobile_bit, fix_wall_bit;
+    // pooled engine: the job's rdf/fast/meso fixes, sampled after the force
+    // pass of every step that is a multiple of rdf_every; rdf_hist[k] holds
+    // rdf_nbin[k] bins and then the sample count
+    int nrdf;
+    int rdf_nbin[POOL_RDF], rdf_every[POOL_RDF], rdf_gi[POOL_RDF], rdf_gj[POOL_RDF];
+    float rdf_rc[POOL_RDF];
+    unsigned *rdf_hist[POOL_RDF];
     float cut_phi, phi_c, phi_factor, dw_factor;
     // pooled engine: Neighbor::decide on the device (src/neighbor.cpp:1216-1290)
     int ago;                     // steps since the last build

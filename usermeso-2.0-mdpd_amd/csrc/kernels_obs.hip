@@ -4,7 +4,8 @@
 // workgroup's bins to the global histogram once (one atomic per bin per
 // workgroup).  Slot nbin of a histogram counts the samples taken; a step
 // whose rows overflowed (the step is repeated once they are widened) is
-// not sampled.  One system (the plain path).
+// not sampled.  k_rdf and k_profile sample one system (the plain path);
+// k_rdf_pool samples each pooled system by its own descriptors.
 //
 // Replaces:
 //   MesoFixRDFFast::post_force / gpu_calc_rdf        fix_rdf_fast_meso.cu:102-175
@@ -15,11 +16,45 @@
 namespace mdpd {
 
 constexpr int OBS_BLK = 256;
-constexpr int OBS_MAXBIN = 4096;      // LDS bins per workgroup
 
-// pairs (i in group gi, j in group gj) of the neighbor rows with r < rc, at
-// the positions of the current buffer; bin = floor(r * nbin / rc).  Every
-// pair appears from both sides, as with the reference's full list.
+// pairs (i in group gi, j in group gj) of particle i's neighbor row with
+// r < rc, at the positions of the current buffer, counted into the LDS bins
+// h; bin = floor(r * nbin / rc).  Every pair appears from both sides, as
+// with the reference's full list.
+__device__ inline void rdf_count(const PoolDev &P, const SysDev *d, int i, unsigned *h, int nbin, float rc, int gi,
+                                 int gj)
+{
+    const int *mask = P.mask[d->sela];
+    if (!(mask[i] & gi)) return;
+    const float4 *pos = P.pos[d->sel];
+    const float4 pi = pos[i];
+    const int maxnbr = P.maxnbr;
+    const int cnt = min(P.ncount[i], maxnbr);
+    const int *row = P.nlist + nbr_at(i, 0, maxnbr);
+    const float rc2 = rc * rc, binv = (float)nbin / rc;
+    for (int q = 0; q < cnt; ++q) {
+        const uint32_t e = (uint32_t)row[(size_t)(q & ~(NBR_CHUNK - 1)) * WAVE + (q & (NBR_CHUNK - 1))];
+        const uint32_t j = e & IDX_MASK;
+        if (!(mask[j] & gj)) continue;
+        const float4 pj = pos[j];
+        float xj = pj.x, yj = pj.y, zj = pj.z;
+        const uint32_t code = e >> IMG_SHIFT;
+        if (code != IMG_ZERO) {
+            int sx, sy, sz;
+            image_decode(code, sx, sy, sz);
+            xj += (float)sx * d->L[0];
+            yj += (float)sy * d->L[1];
+            zj += (float)sz * d->L[2];
+        }
+        const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+        const float rsq = dx * dx + dy * dy + dz * dz;
+        if (rsq < rc2) {
+            const int b = min((int)floorf(rsq * __builtin_amdgcn_rsqf(rsq) * binv), nbin - 1);
+            atomicAdd(&h[b], 1u);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(OBS_BLK) k_rdf(PoolDev P, unsigned *hist, int nbin, float rc, int gi, int gj)
 {
     if (P.sys->overflow) return;
@@ -31,40 +66,44 @@ __global__ void __launch_bounds__(OBS_BLK) k_rdf(PoolDev P, unsigned *hist, int 
     if (i < P.nact) {
         const int s = P.wave_sys[i >> 6];
         const SysDev *d = s >= 0 ? P.sys + s : nullptr;
-        if (d && d->active && i < d->part_base + d->n && (P.mask[d->sela][i] & gi)) {
-            const float4 *pos = P.pos[d->sel];
-            const int *mask = P.mask[d->sela];
-            const float4 pi = pos[i];
-            const int maxnbr = P.maxnbr;
-            const int cnt = min(P.ncount[i], maxnbr);
-            const int *row = P.nlist + nbr_at(i, 0, maxnbr);
-            const float rc2 = rc * rc, binv = (float)nbin / rc;
-            for (int q = 0; q < cnt; ++q) {
-                const uint32_t e = (uint32_t)row[(size_t)(q & ~(NBR_CHUNK - 1)) * WAVE + (q & (NBR_CHUNK - 1))];
-                const uint32_t j = e & IDX_MASK;
-                if (!(mask[j] & gj)) continue;
-                const float4 pj = pos[j];
-                float xj = pj.x, yj = pj.y, zj = pj.z;
-                const uint32_t code = e >> IMG_SHIFT;
-                if (code != IMG_ZERO) {
-                    int sx, sy, sz;
-                    image_decode(code, sx, sy, sz);
-                    xj += (float)sx * d->L[0];
-                    yj += (float)sy * d->L[1];
-                    zj += (float)sz * d->L[2];
-                }
-                const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-                const float rsq = dx * dx + dy * dy + dz * dz;
-                if (rsq < rc2) {
-                    const int b = min((int)floorf(rsq * __builtin_amdgcn_rsqf(rsq) * binv), nbin - 1);
-                    atomicAdd(&h[b], 1u);
-                }
-            }
-        }
+        if (d && d->active && i < d->part_base + d->n) rdf_count(P, d, i, h, nbin, rc, gi, gj);
     }
     __syncthreads();
     for (int b = threadIdx.x; b < nbin; b += OBS_BLK)
         if (h[b]) atomicAdd(hist + b, h[b]);
+}
+
+// Pooled systems: a workgroup covers 256 slots of one system (particle
+// ranges and their padding are multiples of 256) and samples each of its
+// rdf fixes whose `every` divides the step this pass evaluated (PH_RUN and
+// PH_LAST passes: the step is d->step until the commit; the setup pass of a
+// fresh system is not sampled, as a run does not sample its setup).  The
+// workgroup at the system's first slot counts the sample.
+__global__ void __launch_bounds__(OBS_BLK) k_rdf_pool(PoolDev P)
+{
+    __shared__ unsigned h[OBS_MAXBIN];
+    const int i0 = blockIdx.x * OBS_BLK;
+    if (i0 >= P.nact) return;
+    const int s = P.wave_sys[i0 >> 6];
+    if (s < 0) return;
+    const SysDev *d = P.sys + s;
+    if (!d->active || d->overflow || d->nrdf <= 0) return;
+    if (d->phase != PH_RUN && d->phase != PH_LAST) return;
+    const int i = i0 + threadIdx.x;
+    const bool mine = i < d->part_base + d->n && P.wave_sys[i >> 6] == s;
+    for (int k = 0; k < d->nrdf && k < POOL_RDF; ++k) {
+        if (d->step % d->rdf_every[k]) continue;                // uniform over the workgroup
+        const int nbin = min(d->rdf_nbin[k], OBS_MAXBIN);
+        unsigned *hist = d->rdf_hist[k];
+        if (i0 == d->part_base && threadIdx.x == 0) hist[nbin] += 1u;
+        for (int b = threadIdx.x; b < nbin; b += OBS_BLK) h[b] = 0u;
+        __syncthreads();
+        if (mine) rdf_count(P, d, i, h, nbin, d->rdf_rc[k], d->rdf_gi[k], d->rdf_gj[k]);
+        __syncthreads();
+        for (int b = threadIdx.x; b < nbin; b += OBS_BLK)
+            if (h[b]) atomicAdd(hist + b, h[b]);
+        __syncthreads();
+    }
 }
 
 // particles of group g binned by coordinate `axis` of the current buffer:
@@ -98,6 +137,13 @@ hipError_t launch_rdf(const PoolDev &P, unsigned *hist, int nbin, float rc, int 
     if (nbin < 1 || nbin > OBS_MAXBIN) return hipErrorInvalidValue;
     if (P.nact == 0) return hipSuccess;
     hipLaunchKernelGGL(k_rdf, dim3((P.nact + OBS_BLK - 1) / OBS_BLK), dim3(OBS_BLK), 0, st, P, hist, nbin, rc, gi, gj);
+    return hipGetLastError();
+}
+
+hipError_t launch_rdf_pool(const PoolDev &P, hipStream_t st)
+{
+    if (P.nact == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rdf_pool, dim3((P.nact + OBS_BLK - 1) / OBS_BLK), dim3(OBS_BLK), 0, st, P);
     return hipGetLastError();
 }
 

@@ -124,6 +124,15 @@ struct Job {
     std::vector<float4> hp, hv;           // packed input, npad slots
     std::vector<int> ht, hm, ho, himg;
     std::vector<float> hph;               // wall fraction carried in (fix aboundary/rho/meso)
+    std::vector<FixDesc> rdf;             // its rdf/fast/meso fixes (their counts follow the
+    double rdf_cut = 0;                   // harvested block), over pairs r < rdf_cut
+    size_t rdf_bytes() const
+    {
+        size_t b = 0;
+        for (auto &f : rdf) b += sizeof(unsigned) * (size_t)(f.nbin + 1);
+        return b;
+    }
+    size_t result_bytes() const { return harvest_bytes(npad) + rdf_bytes(); }
     // placement while admitted
     int slot = -1;
     long long part_base = -1, cell_base = -1;
@@ -204,6 +213,7 @@ struct mdpd_pool {
     DevBuf<int2> cell_lse;
     DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl, wall_near;
     DevBuf<SysDev> dsys;
+    DevBuf<unsigned> rdf_arena;           // POOL_RDF histograms of OBS_MAXBIN + 1 per descriptor slot
     RangeAlloc parts, cells;              // parts in units of 256 slots
     std::vector<int> free_slots, slot_job;
 
@@ -230,7 +240,7 @@ struct mdpd_pool {
     long long captures = 0;           // graphs of the pass captured so far
     int passes_per_poll = 8;
     std::set<std::pair<int, int>> combos;  // (style, single type) of admitted jobs
-    bool with_rho = false, with_arb = false;
+    bool with_rho = false, with_arb = false, with_rdf = false;
 
     // ---- status polling and staging ----
     int *pin_status[2] = {nullptr, nullptr};
@@ -358,11 +368,18 @@ struct mdpd_pool {
         if (!sys) fail(MDPD_E_ARG, "null system");
         if (nsteps < 1) fail(MDPD_E_ARG, "a job needs at least one step");
         if (!sys->run_style_ok) fail(MDPD_E_STATE, "run_style mvv/meso must be set for the USER-MESO styles");
-        for (auto &f : sys->fixes)
-            if (mdpd_system::is_obs(f))
+        std::vector<FixDesc> rdf;
+        for (auto &f : sys->fixes) {
+            if (f.style == "rdf/fast/meso") rdf.push_back(f);
+            else if (mdpd_system::is_obs(f))
                 fail(MDPD_E_UNSUPPORTED, "fix %s (%s) is not available for pool jobs", f.id.c_str(), f.style.c_str());
+        }
+        if (rdf.size() > (size_t)POOL_RDF)
+            fail(MDPD_E_UNSUPPORTED, "a pool job takes at most %d rdf/fast/meso fixes", POOL_RDF);
         if (sys->setup_done) sys->pull_state();
         auto j = std::make_unique<Job>();
+        j->rdf = rdf;
+        j->rdf_cut = sys->max_cut();
         sys->describe(j->ds);
         j->n = j->ds.d.n;
         j->npad = j->ds.d.npad;
@@ -557,6 +574,24 @@ struct mdpd_pool {
         d.sel = 0;
         d.sela = 0;
         d.split_last = split;
+        d.nrdf = (int)j.rdf.size();
+        if (d.nrdf) {
+            // the slot's histograms start from zero (ordered before the
+            // system's first pass on the stream)
+            const size_t per = (size_t)OBS_MAXBIN + 1;
+            if (!rdf_arena.p) rdf_arena.alloc((size_t)max_sys * POOL_RDF * per);
+            unsigned *base = rdf_arena.p + (size_t)s * POOL_RDF * per;
+            hip_check(hipMemsetAsync(base, 0, sizeof(unsigned) * POOL_RDF * per, stream), "memset");
+            for (int k = 0; k < d.nrdf; ++k) {
+                const FixDesc &f = j.rdf[k];
+                d.rdf_nbin[k] = f.nbin;
+                d.rdf_every[k] = f.every;
+                d.rdf_gi[k] = f.groupbit;
+                d.rdf_gj[k] = f.other_bit;
+                d.rdf_rc[k] = (float)j.rdf_cut;
+                d.rdf_hist[k] = base + (size_t)k * per;
+            }
+        }
         const int npad = j.npad, nc1 = j.ncell + 1, ntw = npad / WAVE;
         const size_t ncoef = j.ds.coef.size(), nt = j.ntypes;
         // staging layout (every piece 16-byte aligned)
@@ -623,6 +658,7 @@ struct mdpd_pool {
         if (!combos.count(combo)) { combos.insert(combo); graph_valid = false; }
         if (d.style == STYLE_MDPD && d.rho_bit && !with_rho) { with_rho = true; graph_valid = false; }
         if (d.fix_arb && !with_arb) { with_arb = true; graph_valid = false; }
+        if (d.nrdf && !with_rdf) { with_rdf = true; graph_valid = false; }
         // the host copy of the input is no longer needed
         std::vector<float4>().swap(j.hp); std::vector<float4>().swap(j.hv);
         std::vector<int>().swap(j.ht); std::vector<int>().swap(j.hm);
@@ -714,6 +750,7 @@ struct mdpd_pool {
         mark(PP_FORCE);
         for (auto &c : combos) hip_check(launch_force(P, c.first, MODE_PHASED, c.second != 0, sp, stream), "force");
         if (with_arb) hip_check(launch_aboundary(P, 1, stream), "aboundary");
+        if (with_rdf) hip_check(launch_rdf_pool(P, stream), "rdf");
         mark(PP_COMMIT);
         hip_check(launch_commit_step(P, COMMIT_PHASED, sp, stream), "commit step");
         mark(PP_COUNT);
@@ -928,8 +965,15 @@ struct mdpd_pool {
         int *dst = nlist.p + (size_t)t0 * maxnbr * WAVE;
         const size_t room = (size_t)j.npad * maxnbr * sizeof(int);
         hip_check(launch_harvest(P, j.slot, (int)j.part_base, j.npad, dst, room, stream), "harvest");
-        j.stage = pinned.get(bytes, j.stage_bytes);
+        j.stage = pinned.get(j.result_bytes(), j.stage_bytes);
         hip_check(hipMemcpyAsync(j.stage, dst, bytes, hipMemcpyDeviceToHost, stream), "harvest copy");
+        size_t at = bytes;
+        for (size_t k = 0; k < j.rdf.size(); ++k) {
+            const size_t b = sizeof(unsigned) * (size_t)(j.rdf[k].nbin + 1);
+            const unsigned *src = rdf_arena.p + ((size_t)j.slot * POOL_RDF + k) * ((size_t)OBS_MAXBIN + 1);
+            hip_check(hipMemcpyAsync((char *)j.stage + at, src, b, hipMemcpyDeviceToHost, stream), "harvest copy");
+            at += b;
+        }
         j.ev = take_event();
         hip_check(hipEventRecord(j.ev, stream), "hipEventRecord");
         j.harvesting = true;
@@ -957,7 +1001,7 @@ struct mdpd_pool {
             j.step_end = d.step;
             if (held_pinned + j.stage_bytes > max_held) {
                 const char *b = (const char *)j.stage;
-                j.kept.assign(b, b + harvest_bytes(j.npad));
+                j.kept.assign(b, b + j.result_bytes());
                 pinned.put(j.stage, j.stage_bytes);
                 j.stage = nullptr;
             } else {
@@ -1228,6 +1272,32 @@ int mdpd_pool_thermo(mdpd_pool *pool, int job, double out[4])
             e[i] = h.f[k].w;
         }
         thermo_of(n, j.mass, ty.data(), v.data(), vir.data(), e.data(), j.volume, out);
+    });
+}
+
+int mdpd_pool_fix_result(mdpd_pool *pool, int job, const char *fix_id, int n, double *x, double *value, int *nbin)
+{
+    POOL_GUARD(pool, {
+        if (!fix_id || !nbin) fail(MDPD_E_ARG, "null argument");
+        Job &j = pool->job(job);
+        size_t at = harvest_bytes(j.npad);
+        const FixDesc *f = nullptr;
+        for (auto &g : j.rdf) {
+            if (g.id == fix_id) { f = &g; break; }
+            at += sizeof(unsigned) * (size_t)(g.nbin + 1);
+        }
+        if (!f) fail(MDPD_E_ARG, "job %d has no rdf/fast/meso fix '%s'", job, fix_id);
+        *nbin = f->nbin;
+        if (n > 0) {
+            if (n < f->nbin || !x || !value) fail(MDPD_E_ARG, "result needs %d bins", f->nbin);
+            if (j.state != JOB_DONE) fail(MDPD_E_STATE, "job %d has not finished", job);
+            const HarvestView h(j);
+            const unsigned *cnt = (const unsigned *)((const char *)j.results() + at);
+            std::vector<double> xs, vs;
+            mdpd_system::rdf_values(*f, cnt, cnt[f->nbin], h.mask, (size_t)j.n, j.volume, j.rdf_cut, xs, vs);
+            std::copy(xs.begin(), xs.end(), x);
+            std::copy(vs.begin(), vs.end(), value);
+        }
     });
 }
 

@@ -1233,6 +1233,31 @@ struct mdpd_system {
         return h;
     }
 
+    // g(r) of fix rdf/fast/meso from its pair counts h[0, nbin) over `samples`
+    // samples, the group sizes counted over the n masks, box volume V and
+    // range rc (fix_rdf_fast_meso.cu:187-217)
+    static void rdf_values(const FixDesc &f, const unsigned *h, long long samples, const int *mask, size_t n,
+                           double V, double rc, std::vector<double> &x, std::vector<double> &val)
+    {
+        const double bin = rc / f.nbin;
+        long long ni = 0, nj = 0;
+        for (size_t i = 0; i < n; ++i) {
+            ni += (mask[i] & f.groupbit) != 0;
+            nj += (mask[i] & f.other_bit) != 0;
+        }
+        const double rho_j = nj / V;
+        x.assign(f.nbin, 0.0);
+        val.assign(f.nbin, 0.0);
+        for (int b = 0; b < f.nbin; ++b) {
+            x[b] = (b + 0.5) * bin;
+            if (!samples || !ni || !nj) continue;
+            const double freq = h[b] / double(ni) / double(samples);
+            const double pi = f.exact_pi ? 3.14159265358979323846 : 3.1415;
+            const double shell = 4. / 3. * pi * (std::pow(bin * (b + 1), 3.) - std::pow(bin * b, 3.));
+            val[b] = freq / shell / rho_j;
+        }
+    }
+
     // current result of an observable fix: bin centres and values (rdf: g(r)
     // with the reference's normalization, 4/3 * 3.1415 * shell volume,
     // fix_rdf_fast_meso.cu:187-217; density: counts per bin averaged over
@@ -1245,22 +1270,8 @@ struct mdpd_system {
         x.assign(f.nbin, 0.0);
         val.assign(f.nbin, 0.0);
         if (f.style == "rdf/fast/meso") {
-            const double rc = max_cut(), bin = rc / f.nbin;
-            long long ni = 0, nj = 0;
-            for (size_t i = 0; i < h_mask.size(); ++i) {
-                ni += (h_mask[i] & f.groupbit) != 0;
-                nj += (h_mask[i] & f.other_bit) != 0;
-            }
             const double V = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
-            const double rho_j = nj / V;
-            for (int b = 0; b < f.nbin; ++b) {
-                x[b] = (b + 0.5) * bin;
-                if (!samples || !ni || !nj) continue;
-                const double freq = h[b] / double(ni) / double(samples);
-                const double pi = f.exact_pi ? 3.14159265358979323846 : 3.1415;
-                const double shell = 4. / 3. * pi * (std::pow(bin * (b + 1), 3.) - std::pow(bin * b, 3.));
-                val[b] = freq / shell / rho_j;
-            }
+            rdf_values(f, h.data(), samples, h_mask.data(), h_mask.size(), V, max_cut(), x, val);
         } else {
             const double bin = (hi[f.along] - lo[f.along]) / f.nbin;
             for (int b = 0; b < f.nbin; ++b) {

@@ -103,6 +103,7 @@ def load():
         "mdpd_pool_done": (I, [P, P, I, ctypes.POINTER(I)]),
         "mdpd_pool_gather": (I, [P, I, ctypes.c_char_p, I, I, P]),
         "mdpd_pool_thermo": (I, [P, I, ctypes.POINTER(D)]),
+        "mdpd_pool_fix_result": (I, [P, I, ctypes.c_char_p, I, P, P, ctypes.POINTER(I)]),
         "mdpd_pool_job_info": (I, [P, I, ctypes.POINTER(LL), ctypes.POINTER(LL), ctypes.POINTER(D)]),
         "mdpd_pool_release": (I, [P, I]),
         "mdpd_pool_tune": (I, [P, ctypes.c_char_p, I]),
@@ -349,6 +350,16 @@ class Pool:
         out = (ctypes.c_double * 4)()
         self._check(self._lib.mdpd_pool_thermo(self._h, int(job), out))
         return {"temp": out[0], "press": out[1], "pe": out[2], "ke": out[3]}
+
+    def fix_result(self, job, fix_id):
+        """(bin centres, g(r)) of a finished job's rdf/fast/meso fix."""
+        nb = ctypes.c_int()
+        self._check(self._lib.mdpd_pool_fix_result(self._h, int(job), fix_id.encode(), 0, None, None, ctypes.byref(nb)))
+        x = np.empty(nb.value)
+        v = np.empty(nb.value)
+        self._check(self._lib.mdpd_pool_fix_result(self._h, int(job), fix_id.encode(), nb.value, _ptr(x), _ptr(v),
+                                                   ctypes.byref(nb)))
+        return x, v
 
     def info(self, job):
         n = ctypes.c_longlong()
