@@ -1852,14 +1852,19 @@ struct mdpd_system {
 
     // a step the graph does not replay because of its observables: a density
     // window is written, more than 32 observable fixes, or a sample pattern
-    // beyond the OBS_VARIANTS graphs kept
+    // beyond the OBS_VARIANTS graphs kept; `fresh` collects the patterns the
+    // current batch will capture
     static constexpr size_t OBS_VARIANTS = 16;
-    bool obs_direct(long long t) const
+    bool obs_direct(long long t, std::vector<uint32_t> &fresh) const
     {
         if (obs.empty()) return false;
         if (obs.size() > 32 || obs_dump_due(t)) return true;
         const uint32_t mask = obs_sample_mask(t);
-        return mask && graph_valid && !gobs.count(mask) && gobs.size() >= OBS_VARIANTS;
+        if (!mask || (graph_valid && gobs.count(mask))) return false;
+        if (std::find(fresh.begin(), fresh.end(), mask) != fresh.end()) return false;
+        if ((graph_valid ? gobs.size() : 0) + fresh.size() >= OBS_VARIANTS) return true;
+        fresh.push_back(mask);
+        return false;
     }
 
     // Drain the stream and read the device step counter; rows that
@@ -1900,7 +1905,8 @@ struct mdpd_system {
         while (k < nsteps) {
             const long long t = t0 + k;
             const bool out = !dumps.empty() && dump_due(t);
-            if (out || obs_direct(t) || prof || !use_graph) {
+            std::vector<uint32_t> fresh;
+            if (out || obs_direct(t, fresh) || prof || !use_graph) {
                 // a step with host work: its rebuild settles before the passes
                 rebuild_phase();
                 if (settle() != t) fail(MDPD_E_HIP, "device step counter out of order at step %lld", t);
@@ -1910,7 +1916,7 @@ struct mdpd_system {
                 continue;
             }
             long long m = 1;
-            while (m < STEP_BATCH && k + m < nsteps && !(!dumps.empty() && dump_due(t + m)) && !obs_direct(t + m))
+            while (m < STEP_BATCH && k + m < nsteps && !(!dumps.empty() && dump_due(t + m)) && !obs_direct(t + m, fresh))
                 ++m;
             // step t+q replays the graph whose observables sample t+q; a step
             // that overflows skips the rest of the batch, samples included
