@@ -238,16 +238,26 @@ int mdpd_pool_gather(mdpd_pool *pool, int job, const char *name, int type, int c
 /* {T, P, pe, ke} of a finished job, as mdpd_thermo. */
 int mdpd_pool_thermo(mdpd_pool *pool, int job, double out[4]);
 
-/* Result of a finished job's fix rdf/fast/meso, as mdpd_fix_result returns
- * it for a handle that ran the job (same bins, counts and normalization).
- * No counterpart in the reference (LAMMPS runs one system); the fix samples
- * each pooled job after its force pass of every step that is a multiple of
- * `every` (fix_rdf_fast_meso.cu:140-175).  At most POOL_RDF = 2 such fixes
- * per job; the output file is not written for pool jobs, and fix
- * density/meso is not available for them.  The histograms take 32 KiB of
- * device memory per resident-job slot, outside the HBM budget, from the
- * first job that has the fix. */
+/* Result of a finished job's observable fix.  No counterpart in the
+ * reference (LAMMPS runs one system).  fix rdf/fast/meso: g(r) as
+ * mdpd_fix_result returns it for a handle that ran the job (same bins,
+ * counts and normalization; sampled after the force pass of every step that
+ * is a multiple of `every`, fix_rdf_fast_meso.cu:140-175).  fix density/meso:
+ * the profile of the job's last window (see mdpd_pool_fix_windows).  A job
+ * takes at most two rdf fixes and one density fix whose windows x (nbin + 1)
+ * stay within 8,192 words; pool jobs write no output files.  The counts take
+ * 64 KiB of device memory per resident-job slot, outside the HBM budget,
+ * from the first job that has such a fix. */
 int mdpd_pool_fix_result(mdpd_pool *pool, int job, const char *fix_id, int n, double *x, double *value, int *nbin);
+
+/* Every window of a finished job's fix density/meso: centre[w] is the
+ * window's centre step (the step in the name of the file a handle writes,
+ * fix_density_profile_meso.cpp:192-215) and value[w * nbin + b] the mean
+ * count in bin b over the window's samples.  *nwin and *nbin receive the
+ * sizes; with max_windows = 0 only they are written.  No counterpart in the
+ * reference. */
+int mdpd_pool_fix_windows(mdpd_pool *pool, int job, const char *fix_id, int max_windows, long long *centre,
+                          double *value, int *nwin, int *nbin);
 /* Atoms, final timestep, and submit / admit / done times (seconds on
  * mdpd_clock) of a job. */
 int mdpd_pool_job_info(mdpd_pool *pool, int job, long long *natoms, long long *timestep, double times[3]);

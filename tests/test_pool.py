@@ -355,9 +355,46 @@ def test_rdf_of_pool_jobs_equals_plain_runs(engine, tmp_path):
             assert g.max() > 0, (t, f)
     with pytest.raises(RuntimeError, match="no rdf"):
         p.fix_result(3, "g")
-    # density profiles and a third rdf fix are refused at submission
-    with pytest.raises(RuntimeError, match="not available for pool jobs"):
-        p.submit(droplet(engine, 2000, 74, extra=f"fix p all density/meso output {tmp_path}/p along z nbin 4 every 10 window 2"), 5)
+    # a third rdf fix, a second density fix and too many windows are refused
     three = "\n".join(f"fix r{k} all rdf/fast/meso output {tmp_path}/r{k} nbin 8" for k in range(3))
     with pytest.raises(RuntimeError, match="at most 2"):
         p.submit(droplet(engine, 2000, 75, extra=three), 5)
+    two = "\n".join(f"fix p{k} all density/meso output {tmp_path}/p{k} along z nbin 4 every 10 window 2" for k in range(2))
+    with pytest.raises(RuntimeError, match="at most one"):
+        p.submit(droplet(engine, 2000, 76, extra=two), 5)
+    with pytest.raises(RuntimeError, match="exceed"):
+        p.submit(droplet(engine, 2000, 77, extra=f"fix p all density/meso output {tmp_path}/q along z nbin 100 every 10 window 2"), 1000)
+
+
+def test_density_windows_of_pool_jobs_equal_the_files_of_plain_runs(engine, tmp_path):
+    # density/meso on pooled jobs: every window's profile equals the file a
+    # handle writes for it after the same run, and the last window equals the
+    # handle's current result (job a: four whole windows; job b ends inside
+    # its last window, which the handle has not written)
+    specs = [("a", 2000, 81, 70, "along z nbin 6 every 20 window 3", [0, 20, 40, 60]),
+             ("b", 2600, 82, 27, "along x nbin 5 every 7 window 2", [0, 7, 14, 21, 28])]
+    want = {}
+    for t, n, seed, steps, args, centres in specs:
+        d = tmp_path / t
+        d.mkdir()
+        s = droplet(engine, n, seed, extra=f"fix p all density/meso output {d}/prof {args}")
+        s.tune("split", 2)
+        s.run(steps)
+        want[t] = ([np.loadtxt(d / f"prof.{m:09d}")[:, 1] for m in centres if (d / f"prof.{m:09d}").exists()],
+                   s.fix_result("p"))
+        s.close()
+    got, p = pool_results(engine, [droplet(engine, n, seed, extra=f"fix p all density/meso output {tmp_path}/x {args}")
+                                   for _, n, seed, _, args, _ in specs], [sp[3] for sp in specs])
+    for k, (t, n, seed, steps, args, centres) in enumerate(specs):
+        c, v = p.fix_windows(k + 1, "p")
+        assert c.tolist() == centres, (t, c)
+        files, (x, last) = want[t]
+        assert len(files) >= len(centres) - 1
+        for w, f in enumerate(files):
+            np.testing.assert_allclose(v[w], f, rtol=1e-14, atol=0, err_msg=f"{t} window {centres[w]}")
+            assert f.sum() > 0
+        xp, lp = p.fix_result(k + 1, "p")
+        np.testing.assert_array_equal(xp, x)
+        if t == "b":
+            np.testing.assert_array_equal(lp, last)             # the handle's current window
+        np.testing.assert_array_equal(lp, v[-1])

@@ -12,6 +12,7 @@ constexpr int WAVE = 64;
 constexpr int MAXTYPES = 8;
 constexpr int POOL_RDF = 2;        // rdf/fast/meso fixes per pooled job
 constexpr int OBS_MAXBIN = 4096;   // bins of an observable histogram
+constexpr int POOL_DENS_WORDS = 8192;   // density windows x (bins + 1) per pooled job
 constexpr int NCOEF = 10;          // per type pair, see Coef
 constexpr int IMG_SHIFT = 27;      // neighbor entry = j | image << 27
 constexpr uint32_t IDX_MASK = (1u << IMG_SHIFT) - 1u;
@@ -149,6 +150,12 @@ struct SysDev {
     int rdf_nbin[POOL_RDF], rdf_every[POOL_RDF], rdf_gi[POOL_RDF], rdf_gj[POOL_RDF];
     float rdf_rc[POOL_RDF];
     unsigned *rdf_hist[POOL_RDF];
+    // pooled engine: the job's density/meso fix; window w (centre dens_m0 +
+    // w * dens_every) keeps dens_nbin counts and then its sample count at
+    // dens_hist + w * (dens_nbin + 1)
+    int ndens, dens_nbin, dens_every, dens_window, dens_along, dens_g, dens_nwin;
+    long long dens_m0;
+    unsigned *dens_hist;
     float cut_phi, phi_c, phi_factor, dw_factor;
     // pooled engine: Neighbor::decide on the device (src/neighbor.cpp:1216-1290)
     int ago;                     // steps since the last build

@@ -57,8 +57,9 @@ hipError_t launch_harvest(const PoolDev &P, int s, int part_base, int npad, void
 // counting the samples; a system whose rows overflowed is not sampled)
 hipError_t launch_rdf(const PoolDev &P, unsigned *hist, int nbin, float rc, int gi, int gj, hipStream_t st);
 hipError_t launch_profile(const PoolDev &P, unsigned *count, int nbin, int axis, int g, hipStream_t st);
-// rdf/fast/meso of every pooled system that has it and samples this pass
-hipError_t launch_rdf_pool(const PoolDev &P, hipStream_t st);
+// rdf/fast/meso and density/meso of every pooled system that has them and
+// samples this pass
+hipError_t launch_obs_pool(const PoolDev &P, hipStream_t st);
 
 // stateless helpers for parity tests
 hipError_t launch_pair_gaussian(const uint32_t *a, const uint32_t *b, float *out, int n, hipStream_t st);

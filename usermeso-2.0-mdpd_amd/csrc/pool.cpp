@@ -120,19 +120,24 @@ struct Job {
     int n = 0, npad = 0, ncell = 0, ntypes = 0;
     long long nsteps = 0;
     double volume = 0;
+    double box[3] = {0, 0, 0};            // edge lengths
     double mass[MAXTYPES + 1] = {0};
     std::vector<float4> hp, hv;           // packed input, npad slots
     std::vector<int> ht, hm, ho, himg;
     std::vector<float> hph;               // wall fraction carried in (fix aboundary/rho/meso)
     std::vector<FixDesc> rdf;             // its rdf/fast/meso fixes (their counts follow the
     double rdf_cut = 0;                   // harvested block), over pairs r < rdf_cut
+    std::vector<FixDesc> dens;            // its density/meso fix (at most one): windows
+    long long dens_m0 = 0;                // centred at dens_m0 + w * every, w < dens_nwin,
+    int dens_nwin = 0;                    // whose counts follow the rdf counts
     size_t rdf_bytes() const
     {
         size_t b = 0;
         for (auto &f : rdf) b += sizeof(unsigned) * (size_t)(f.nbin + 1);
         return b;
     }
-    size_t result_bytes() const { return harvest_bytes(npad) + rdf_bytes(); }
+    size_t dens_bytes() const { return dens.empty() ? 0 : sizeof(unsigned) * (size_t)dens_nwin * (dens[0].nbin + 1); }
+    size_t result_bytes() const { return harvest_bytes(npad) + rdf_bytes() + dens_bytes(); }
     // placement while admitted
     int slot = -1;
     long long part_base = -1, cell_base = -1;
@@ -188,6 +193,9 @@ constexpr double BYTES_PER_ENTRY = 8.0;
 constexpr int CELLS_PER_SLOT = 2;
 constexpr long long RETUNE_PASSES = 256;
 constexpr size_t ADMIT_LOOKAHEAD = 64;    // queued jobs considered per admission round
+// observable words per descriptor slot: the rdf histograms, then the
+// density windows
+constexpr size_t SLOT_OBS_WORDS = (size_t)POOL_RDF * (OBS_MAXBIN + 1) + POOL_DENS_WORDS;
 constexpr int MAX_BATCH = 16;             // passes captured in one batch graph
 constexpr int MAX_BYPASS = 64;            // admissions past one waiting head job
 
@@ -213,7 +221,8 @@ struct mdpd_pool {
     DevBuf<int2> cell_lse;
     DevBuf<int> nlist, ncount, clist, ccount, wave_sys, moved, status, cl, wall_near;
     DevBuf<SysDev> dsys;
-    DevBuf<unsigned> rdf_arena;           // POOL_RDF histograms of OBS_MAXBIN + 1 per descriptor slot
+    DevBuf<unsigned> obs_arena;           // per descriptor slot: POOL_RDF histograms of OBS_MAXBIN + 1
+                                          // words, then POOL_DENS_WORDS of density windows
     RangeAlloc parts, cells;              // parts in units of 256 slots
     std::vector<int> free_slots, slot_job;
 
@@ -240,7 +249,7 @@ struct mdpd_pool {
     long long captures = 0;           // graphs of the pass captured so far
     int passes_per_poll = 8;
     std::set<std::pair<int, int>> combos;  // (style, single type) of admitted jobs
-    bool with_rho = false, with_arb = false, with_rdf = false;
+    bool with_rho = false, with_arb = false, with_obs = false;
 
     // ---- status polling and staging ----
     int *pin_status[2] = {nullptr, nullptr};
@@ -368,18 +377,41 @@ struct mdpd_pool {
         if (!sys) fail(MDPD_E_ARG, "null system");
         if (nsteps < 1) fail(MDPD_E_ARG, "a job needs at least one step");
         if (!sys->run_style_ok) fail(MDPD_E_STATE, "run_style mvv/meso must be set for the USER-MESO styles");
-        std::vector<FixDesc> rdf;
+        std::vector<FixDesc> rdf, dens;
         for (auto &f : sys->fixes) {
             if (f.style == "rdf/fast/meso") rdf.push_back(f);
-            else if (mdpd_system::is_obs(f))
-                fail(MDPD_E_UNSUPPORTED, "fix %s (%s) is not available for pool jobs", f.id.c_str(), f.style.c_str());
+            else if (f.style == "density/meso") dens.push_back(f);
         }
         if (rdf.size() > (size_t)POOL_RDF)
             fail(MDPD_E_UNSUPPORTED, "a pool job takes at most %d rdf/fast/meso fixes", POOL_RDF);
+        if (dens.size() > 1) fail(MDPD_E_UNSUPPORTED, "a pool job takes at most one density/meso fix");
+        // the density windows the job samples: steps t0+1 .. t0+nsteps, each
+        // window of `every` steps holds samples (window >= 1)
+        long long m0 = 0;
+        int nwin = 0;
+        if (!dens.empty()) {
+            const FixDesc &f = dens[0];
+            const long long t0 = sys->ntimestep, t1 = t0 + nsteps, ev = f.every;
+            auto centre = [&](long long t) { return (t + ev / 2) / ev * ev; };
+            long long a = t0 + 1, b = t1;
+            while (a <= t1 && !mdpd_system::density_sample(f, a)) ++a;
+            while (b > a && !mdpd_system::density_sample(f, b)) --b;
+            if (a <= t1) {
+                m0 = centre(a);
+                const long long nw = (centre(b) - m0) / ev + 1;
+                if (nw * (f.nbin + 1) > POOL_DENS_WORDS)
+                    fail(MDPD_E_UNSUPPORTED, "fix %s: %lld windows of %d bins exceed the %d words a pool job keeps",
+                         f.id.c_str(), nw, f.nbin, POOL_DENS_WORDS);
+                nwin = (int)nw;
+            }
+        }
         if (sys->setup_done) sys->pull_state();
         auto j = std::make_unique<Job>();
         j->rdf = rdf;
         j->rdf_cut = sys->max_cut();
+        j->dens = dens;
+        j->dens_m0 = m0;
+        j->dens_nwin = nwin;
         sys->describe(j->ds);
         j->n = j->ds.d.n;
         j->npad = j->ds.d.npad;
@@ -387,6 +419,7 @@ struct mdpd_pool {
         j->ntypes = sys->ntypes;
         j->nsteps = nsteps;
         j->volume = (sys->hi[0] - sys->lo[0]) * (sys->hi[1] - sys->lo[1]) * (sys->hi[2] - sys->lo[2]);
+        for (int k = 0; k < 3; ++k) j->box[k] = sys->hi[k] - sys->lo[k];
         for (int t = 0; t <= MAXTYPES; ++t) j->mass[t] = sys->mass[t];
         const int mnb = std::max(maxnbr, j->ds.want_maxnbr);
         const long long c = allocated ? cap : capacity_for(mnb);
@@ -575,13 +608,25 @@ struct mdpd_pool {
         d.sela = 0;
         d.split_last = split;
         d.nrdf = (int)j.rdf.size();
-        if (d.nrdf) {
+        d.ndens = j.dens_nwin > 0 ? 1 : 0;
+        if (d.nrdf || d.ndens) {
             // the slot's histograms start from zero (ordered before the
             // system's first pass on the stream)
             const size_t per = (size_t)OBS_MAXBIN + 1;
-            if (!rdf_arena.p) rdf_arena.alloc((size_t)max_sys * POOL_RDF * per);
-            unsigned *base = rdf_arena.p + (size_t)s * POOL_RDF * per;
-            hip_check(hipMemsetAsync(base, 0, sizeof(unsigned) * POOL_RDF * per, stream), "memset");
+            if (!obs_arena.p) obs_arena.alloc((size_t)max_sys * SLOT_OBS_WORDS);
+            unsigned *base = obs_arena.p + (size_t)s * SLOT_OBS_WORDS;
+            hip_check(hipMemsetAsync(base, 0, sizeof(unsigned) * SLOT_OBS_WORDS, stream), "memset");
+            if (d.ndens) {
+                const FixDesc &f = j.dens[0];
+                d.dens_nbin = f.nbin;
+                d.dens_every = f.every;
+                d.dens_window = f.window;
+                d.dens_along = f.along;
+                d.dens_g = f.groupbit;
+                d.dens_nwin = j.dens_nwin;
+                d.dens_m0 = j.dens_m0;
+                d.dens_hist = base + POOL_RDF * per;
+            }
             for (int k = 0; k < d.nrdf; ++k) {
                 const FixDesc &f = j.rdf[k];
                 d.rdf_nbin[k] = f.nbin;
@@ -658,7 +703,7 @@ struct mdpd_pool {
         if (!combos.count(combo)) { combos.insert(combo); graph_valid = false; }
         if (d.style == STYLE_MDPD && d.rho_bit && !with_rho) { with_rho = true; graph_valid = false; }
         if (d.fix_arb && !with_arb) { with_arb = true; graph_valid = false; }
-        if (d.nrdf && !with_rdf) { with_rdf = true; graph_valid = false; }
+        if ((d.nrdf || d.ndens) && !with_obs) { with_obs = true; graph_valid = false; }
         // the host copy of the input is no longer needed
         std::vector<float4>().swap(j.hp); std::vector<float4>().swap(j.hv);
         std::vector<int>().swap(j.ht); std::vector<int>().swap(j.hm);
@@ -750,7 +795,7 @@ struct mdpd_pool {
         mark(PP_FORCE);
         for (auto &c : combos) hip_check(launch_force(P, c.first, MODE_PHASED, c.second != 0, sp, stream), "force");
         if (with_arb) hip_check(launch_aboundary(P, 1, stream), "aboundary");
-        if (with_rdf) hip_check(launch_rdf_pool(P, stream), "rdf");
+        if (with_obs) hip_check(launch_obs_pool(P, stream), "observables");
         mark(PP_COMMIT);
         hip_check(launch_commit_step(P, COMMIT_PHASED, sp, stream), "commit step");
         mark(PP_COUNT);
@@ -968,12 +1013,17 @@ struct mdpd_pool {
         j.stage = pinned.get(j.result_bytes(), j.stage_bytes);
         hip_check(hipMemcpyAsync(j.stage, dst, bytes, hipMemcpyDeviceToHost, stream), "harvest copy");
         size_t at = bytes;
+        const unsigned *base = obs_arena.p + (size_t)j.slot * SLOT_OBS_WORDS;
         for (size_t k = 0; k < j.rdf.size(); ++k) {
             const size_t b = sizeof(unsigned) * (size_t)(j.rdf[k].nbin + 1);
-            const unsigned *src = rdf_arena.p + ((size_t)j.slot * POOL_RDF + k) * ((size_t)OBS_MAXBIN + 1);
+            const unsigned *src = base + k * ((size_t)OBS_MAXBIN + 1);
             hip_check(hipMemcpyAsync((char *)j.stage + at, src, b, hipMemcpyDeviceToHost, stream), "harvest copy");
             at += b;
         }
+        if (j.dens_bytes())
+            hip_check(hipMemcpyAsync((char *)j.stage + at, base + (size_t)POOL_RDF * (OBS_MAXBIN + 1), j.dens_bytes(),
+                                     hipMemcpyDeviceToHost, stream),
+                      "harvest copy");
         j.ev = take_event();
         hip_check(hipEventRecord(j.ev, stream), "hipEventRecord");
         j.harvesting = true;
@@ -1028,6 +1078,20 @@ struct mdpd_pool {
         }
         if (done) update_extent();
         return done;
+    }
+
+    // a finished job's density windows: centre step and the mean count per
+    // bin over the window's samples (fix_density_profile_meso.cpp:192-215)
+    void density_windows(const Job &j, long long *centre, double *value) const
+    {
+        const int nb = j.dens[0].nbin;
+        const unsigned *c = (const unsigned *)((const char *)j.results() + harvest_bytes(j.npad) + j.rdf_bytes());
+        for (int w = 0; w < j.dens_nwin; ++w) {
+            const unsigned *h = c + (size_t)w * (nb + 1);
+            const unsigned samples = h[nb];
+            if (centre) centre[w] = j.dens_m0 + (long long)w * j.dens[0].every;
+            for (int b = 0; b < nb; ++b) value[(size_t)w * nb + b] = samples ? h[b] / double((float)samples) : 0.0;
+        }
     }
 
     // ------------------------------------------------------------------
@@ -1280,13 +1344,29 @@ int mdpd_pool_fix_result(mdpd_pool *pool, int job, const char *fix_id, int n, do
     POOL_GUARD(pool, {
         if (!fix_id || !nbin) fail(MDPD_E_ARG, "null argument");
         Job &j = pool->job(job);
+        if (!j.dens.empty() && j.dens[0].id == fix_id) {
+            // density/meso: the job's last window
+            *nbin = j.dens[0].nbin;
+            if (n > 0) {
+                if (n < *nbin || !x || !value) fail(MDPD_E_ARG, "result needs %d bins", *nbin);
+                if (j.state != JOB_DONE) fail(MDPD_E_STATE, "job %d has not finished", job);
+                std::vector<double> v((size_t)std::max(j.dens_nwin, 1) * *nbin, 0.0);
+                if (j.dens_nwin) pool->density_windows(j, nullptr, v.data());
+                const double bin = j.box[j.dens[0].along] / *nbin;
+                for (int b = 0; b < *nbin; ++b) {
+                    x[b] = (b + 0.5) * bin;
+                    value[b] = v[(size_t)std::max(j.dens_nwin - 1, 0) * *nbin + b];
+                }
+            }
+            return MDPD_OK;
+        }
         size_t at = harvest_bytes(j.npad);
         const FixDesc *f = nullptr;
         for (auto &g : j.rdf) {
             if (g.id == fix_id) { f = &g; break; }
             at += sizeof(unsigned) * (size_t)(g.nbin + 1);
         }
-        if (!f) fail(MDPD_E_ARG, "job %d has no rdf/fast/meso fix '%s'", job, fix_id);
+        if (!f) fail(MDPD_E_ARG, "job %d has no rdf/fast/meso or density/meso fix '%s'", job, fix_id);
         *nbin = f->nbin;
         if (n > 0) {
             if (n < f->nbin || !x || !value) fail(MDPD_E_ARG, "result needs %d bins", f->nbin);
@@ -1297,6 +1377,23 @@ int mdpd_pool_fix_result(mdpd_pool *pool, int job, const char *fix_id, int n, do
             mdpd_system::rdf_values(*f, cnt, cnt[f->nbin], h.mask, (size_t)j.n, j.volume, j.rdf_cut, xs, vs);
             std::copy(xs.begin(), xs.end(), x);
             std::copy(vs.begin(), vs.end(), value);
+        }
+    });
+}
+
+int mdpd_pool_fix_windows(mdpd_pool *pool, int job, const char *fix_id, int max_windows, long long *centre,
+                          double *value, int *nwin, int *nbin)
+{
+    POOL_GUARD(pool, {
+        if (!fix_id || !nwin || !nbin) fail(MDPD_E_ARG, "null argument");
+        Job &j = pool->job(job);
+        if (j.dens.empty() || j.dens[0].id != fix_id) fail(MDPD_E_ARG, "job %d has no density/meso fix '%s'", job, fix_id);
+        *nwin = j.dens_nwin;
+        *nbin = j.dens[0].nbin;
+        if (max_windows > 0) {
+            if (max_windows < j.dens_nwin || !centre || !value) fail(MDPD_E_ARG, "result needs %d windows", j.dens_nwin);
+            if (j.state != JOB_DONE) fail(MDPD_E_STATE, "job %d has not finished", job);
+            pool->density_windows(j, centre, value);
         }
     });
 }

@@ -104,6 +104,7 @@ def load():
         "mdpd_pool_gather": (I, [P, I, ctypes.c_char_p, I, I, P]),
         "mdpd_pool_thermo": (I, [P, I, ctypes.POINTER(D)]),
         "mdpd_pool_fix_result": (I, [P, I, ctypes.c_char_p, I, P, P, ctypes.POINTER(I)]),
+        "mdpd_pool_fix_windows": (I, [P, I, ctypes.c_char_p, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "mdpd_pool_job_info": (I, [P, I, ctypes.POINTER(LL), ctypes.POINTER(LL), ctypes.POINTER(D)]),
         "mdpd_pool_release": (I, [P, I]),
         "mdpd_pool_tune": (I, [P, ctypes.c_char_p, I]),
@@ -352,7 +353,8 @@ class Pool:
         return {"temp": out[0], "press": out[1], "pe": out[2], "ke": out[3]}
 
     def fix_result(self, job, fix_id):
-        """(bin centres, g(r)) of a finished job's rdf/fast/meso fix."""
+        """(bin centres, values) of a finished job's rdf/fast/meso fix (g(r)) or
+        density/meso fix (the profile of its last window)."""
         nb = ctypes.c_int()
         self._check(self._lib.mdpd_pool_fix_result(self._h, int(job), fix_id.encode(), 0, None, None, ctypes.byref(nb)))
         x = np.empty(nb.value)
@@ -360,6 +362,18 @@ class Pool:
         self._check(self._lib.mdpd_pool_fix_result(self._h, int(job), fix_id.encode(), nb.value, _ptr(x), _ptr(v),
                                                    ctypes.byref(nb)))
         return x, v
+
+    def fix_windows(self, job, fix_id):
+        """(centre steps, profiles [window, bin]) of a finished job's density/meso fix."""
+        nw, nb = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.mdpd_pool_fix_windows(self._h, int(job), fix_id.encode(), 0, None, None,
+                                                    ctypes.byref(nw), ctypes.byref(nb)))
+        c = np.zeros(nw.value, np.int64)
+        v = np.zeros((nw.value, nb.value))
+        if nw.value:
+            self._check(self._lib.mdpd_pool_fix_windows(self._h, int(job), fix_id.encode(), nw.value, _ptr(c), _ptr(v),
+                                                        ctypes.byref(nw), ctypes.byref(nb)))
+        return c, v
 
     def info(self, job):
         n = ctypes.c_longlong()
