@@ -49,7 +49,8 @@ __device__ inline void rdf_count(const PoolDev &P, const SysDev *d, int i, unsig
         const float dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
         const float rsq = dx * dx + dy * dy + dz * dz;
         if (rsq < rc2) {
-            const int b = min((int)floorf(rsq * __builtin_amdgcn_rsqf(rsq) * binv), nbin - 1);
+            const float r = rsq > 0.f ? rsq * __builtin_amdgcn_rsqf(rsq) : 0.f;   // coincident pair: bin 0
+            const int b = min((int)floorf(r * binv), nbin - 1);
             atomicAdd(&h[b], 1u);
         }
     }
